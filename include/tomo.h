@@ -1,0 +1,300 @@
+/*
+ * tomo.h -- C ABI of the MI355X wave-tomography engine (libtomo_hip.so).
+ *
+ * The engine runs the nonlinear wave-based SIRT / ART loop of
+ * crogers719/Tomographic-Image-Reconstruction-Algorithms on one AMD GPU:
+ * 2-D variable-velocity acoustic FDTD forward propagation with an absorbing
+ * edge, sensor-trace record, residual, time-reversed adjoint propagation
+ * with residual injection, forward x adjoint correlation (the gradient
+ * image) and the uniform SIRT, non-uniform weighted SIRT and sequential ART
+ * field updates.
+ *
+ * The reference has no library API: its "boundary" is a set of CUDA
+ * kernels launched in a fixed order by one host function per executable,
+ * plus the RAII buffer types of cuda_ptr.cuh.  Every entry point below names
+ * the reference code it replaces (file:line, paths relative to the
+ * reference repository root).
+ *
+ * Conventions
+ *  - Arrays are row-major with i (x) fastest: a 2-D field is a(i, j) =
+ *    a[i + nx * j], exactly the host_ptr indexing of cuda_ptr.cuh:54.
+ *  - Traces are float[shots][nt][nr]: receiver r of shot s at time level k
+ *    is traces[(s * nt + k) * nr + r].  tomo_traces_to_sides() converts to
+ *    the reference's four per-side arrays (generate-data.cu:187-223).
+ *  - Every function returns an int status: TOMO_OK (0) or a negative
+ *    TOMO_ERR_* code; tomo_last_error() gives the message of the last
+ *    failure on a context.  The caller owns every host buffer passed in;
+ *    the context owns all device memory.  One context = one HIP stream on
+ *    one device; a context is not thread-safe.
+ */
+#ifndef TOMO_H
+#define TOMO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOMO_ABI_VERSION 1
+
+enum tomo_status {
+  TOMO_OK = 0,
+  TOMO_ERR_ARG = -1,    /* bad argument (null pointer, size out of range) */
+  TOMO_ERR_HIP = -2,    /* HIP runtime error */
+  TOMO_ERR_ALLOC = -3,  /* device or host allocation failed */
+  TOMO_ERR_STATE = -4,  /* call made before its prerequisites (no field, no data ...) */
+  TOMO_ERR_IO = -5,     /* file missing / malformed */
+  TOMO_ERR_NODEV = -6   /* no usable GPU */
+};
+
+/* source wavelets */
+enum tomo_wavelet {
+  /* reference pulse cos(wc t) exp(-t^2 / (2 tau^2)), t = k dt - t_shift,
+   * added while k + 1 <= src_last_level (sirt-mimo.cu:553-562) */
+  TOMO_WAVELET_REF_GAUSS_COS = 0,
+  /* Ricker (1 - 2 pi^2 fp^2 t^2) exp(-pi^2 fp^2 t^2), t = k dt - t0 */
+  TOMO_WAVELET_RICKER = 1
+};
+
+/* receiver layouts */
+enum tomo_receivers {
+  /* four sensor lines: rows/cols line_lo and line_hi, positions
+   * line_first..line_last; residual injected by overwrite from the inward
+   * neighbour (sirt-mimo.cu:617-649, 712-740) */
+  TOMO_RECV_LINES = 0,
+  /* point receivers set by tomo_set_receivers(); residual added at the cell */
+  TOMO_RECV_POINTS = 1
+};
+
+/* uniform SIRT update variants */
+enum tomo_uniform_mode {
+  /* f += scale * mask * sum_g G_g / (1 + f)          (sirt-mimo.cu:858-877)   */
+  TOMO_UPDATE_SUM = 0,
+  /* f += scale * mask * (sum_g G_g) / (1 + f) / Ng   (sirt-mimo-io.cu:774-796) */
+  TOMO_UPDATE_MEAN = 1
+};
+
+/* relaxation schedules (host scalars) */
+enum tomo_schedule_kind {
+  /* omega * (alpha + beta / (k + 1)^gamma), k = 0-based iteration
+   * (sirt-mimo-io.cu:355); params = {omega, alpha, beta, gamma} */
+  TOMO_SCHED_POWER = 0,
+  /* k 1-based: odd k -> a1 + b1 / k^n1, even k -> a2 - b2 / k^n2
+   * (sirt-mimo-norm-alt.cu:472-477); params = {a1, b1, n1, a2, b2, n2} */
+  TOMO_SCHED_ALTERNATING = 1
+};
+
+typedef struct tomo_config {
+  int32_t nx, ny;          /* grid cells (i fastest) */
+  int32_t nt;              /* time levels 0..nt-1; levels 0 and 1 are zero */
+  int32_t wavelet;         /* enum tomo_wavelet */
+  double h;                /* cell size */
+  double dt;               /* time step */
+  double c0;               /* background speed, v = c0 sqrt(1 + f) */
+  double wl_omega;         /* gauss-cos: angular frequency (OMEGAC) */
+  double wl_tau;           /* gauss-cos: envelope width (TAO) */
+  double wl_shift;         /* gauss-cos: time shift (TT) */
+  double ricker_fp;        /* Ricker peak frequency */
+  double ricker_t0;        /* Ricker delay */
+  int32_t src_last_level;  /* source active while k + 1 <= this; <= 0: no window */
+  int32_t sponge_width;    /* absorbing sponge cells inside the edge; 0 = edge rule only */
+  double sponge_gamma;     /* peak sponge damping (per-step gamma at the edge) */
+  int32_t recv_mode;       /* enum tomo_receivers */
+  int32_t record_first;    /* first recorded time level (reference: 2) */
+  int32_t line_lo, line_hi;       /* TOMO_RECV_LINES: the two line rows/cols */
+  int32_t line_first, line_last;  /* TOMO_RECV_LINES: inclusive span along a line */
+  double recv_gain;        /* adjoint injection factor (reference: H * 1000) */
+  int32_t mask_x0, mask_x1, mask_y0, mask_y1;  /* update mask [x0,x1) x [y0,y1) */
+  int32_t checkpoint_every;  /* 0: keep the whole forward history; C > 0: keep
+                                every C-th level pair and recompute segments */
+  int32_t shots_per_pass;    /* 0: all shots per pass; P: P shots per pass
+                                (memory-scalable SIRT, sirt-mimo-norm-mem.cu:429) */
+} tomo_config;
+
+typedef struct tomo_ctx tomo_ctx;
+
+/* ---------------------------------------------------------------- config */
+
+/* Reference-parity preset: 192 x 192, NT = 401, H = 1e-3, DT = 3.3333e-7,
+ * c0 = 1500, gauss-cos pulse cut at k + 1 <= 24, edge rule only, four sensor
+ * lines at 20 / 180 spanning 21..179, record from level 2, gain H * 1000,
+ * mask [21,180)^2, full history.  Replaces the #defines of
+ * sirt-mimo.cu:17-35 and the literals of sirt-mimo.cu:525, 629, 724, 870. */
+void tomo_config_reference(tomo_config *cfg);
+
+/* Unit-spacing preset for an n x n grid and nt levels: h = 1, c0 = 1,
+ * dt = 0.4 (CFL 0.4), Ricker fp = 0.1 (10 cells per wavelength), t0 = 1.5/fp,
+ * 20-cell sponge, point receivers, mask = central n/2 square.  No
+ * counterpart in the reference (it hard-codes the preset above). */
+void tomo_config_unit(tomo_config *cfg, int n, int nt);
+
+/* ------------------------------------------------------------- lifecycle */
+
+/* Replaces the device_ptr allocations of Ultrasonic_Tomography()
+ * (sirt-mimo-io.cu:97-231).  device < 0 picks the current device. */
+int tomo_create(const tomo_config *cfg, int device, tomo_ctx **out);
+void tomo_destroy(tomo_ctx *ctx);
+const char *tomo_last_error(const tomo_ctx *ctx);
+const char *tomo_strerror(int status);
+int tomo_abi_version(void);
+
+/* ------------------------------------------------------------- geometry */
+
+/* Reference transducer ring: 640 positions on the square [21,181]^2, 160 per
+ * side, bottom -> right -> top -> left, 1-based (Position_Transducers,
+ * sirt-mimo.cu:438-466).  ns <= 640. */
+int tomo_transducers_reference(int *ii, int *jj, int ns);
+
+/* count transducers evenly on a circle of radius `radius` cells centred on
+ * the grid centre, 0-based cell indices.  No reference counterpart. */
+int tomo_transducers_ring(int nx, int ny, int count, double radius, int *ii, int *jj);
+
+/* Set the transducer positions (one_based: 1 for the reference convention). */
+int tomo_set_transducers(tomo_ctx *ctx, const int *ii, const int *jj, int ns, int one_based);
+
+/* Consecutive source groups of `group_size` transducers; group g fires the
+ * filled bounding box of transducers g*G and g*G+G-1 (sirt-mimo.cu:157 and
+ * 543-554).  Returns the number of groups (shots) or a negative status. */
+int tomo_set_groups(tomo_ctx *ctx, int group_size);
+
+/* TOMO_RECV_POINTS only: receiver cells (one_based as above). */
+int tomo_set_receivers(tomo_ctx *ctx, const int *ri, const int *rj, int nr, int one_based);
+
+int tomo_num_shots(const tomo_ctx *ctx);
+int tomo_num_receivers(const tomo_ctx *ctx);
+/* receiver cells as 0-based (i, j) pairs, for trace layout conversions */
+int tomo_get_receivers(const tomo_ctx *ctx, int *ri, int *rj);
+
+/* ---------------------------------------------------------------- fields */
+
+/* Image field f = c^2 / c0^2 - 1, float[ny][nx]. */
+int tomo_set_field(tomo_ctx *ctx, const float *f);
+int tomo_get_field(tomo_ctx *ctx, float *f);
+/* ground truth fo for the error epsilon (sirt-mimo-io.cu:167-168) */
+int tomo_set_truth(tomo_ctx *ctx, const float *fo);
+
+/* Reference phantom: disk r = 15 mm at 0.03 plus three r = 5 mm disks at
+ * 0.05, in float arithmetic (field_setup, sirt-mimo.cu:470-509;
+ * generate-field.cu:117-172).  x = -0.1 + i * h. */
+int tomo_phantom_reference(float *fo, int nx, int ny, double h);
+
+/* Seeded circular inclusions in a zero background inside the region
+ * [x0,x1) x [y0,y1): count disks, radius uniform in [rmin, rmax] cells,
+ * velocity contrast uniform in +-[cmin, cmax]; f = (1 + contrast)^2 - 1.
+ * Added on top of the existing values of fo.  No reference counterpart. */
+int tomo_phantom_inclusions(float *fo, int nx, int ny, uint64_t seed, int count,
+                            double rmin, double rmax, double cmin, double cmax,
+                            int x0, int x1, int y0, int y1);
+
+/* Seeded smooth random velocity: white noise filtered by a Gaussian of
+ * width sigma cells, scaled to a peak contrast of +-amp, f = (1+c)^2 - 1,
+ * confined to [x0,x1) x [y0,y1).  No reference counterpart. */
+int tomo_phantom_smooth(float *fo, int nx, int ny, uint64_t seed, double sigma, double amp,
+                        int x0, int x1, int y0, int y1);
+
+/* ------------------------------------------------------ forward / data */
+
+/* Forward-propagate shots [g0, g0+ng) on the current field and return their
+ * traces float[ng][nt][nr] (propagation + propagation_at_corners +
+ * initial_signal, generate-data.cu:157-195).  fp64 != 0 runs the FP64
+ * solver (used to make measured data). */
+int tomo_forward(tomo_ctx *ctx, int g0, int ng, int fp64, float *traces);
+
+/* Measured traces float[nshots][nt][nr] (sirt-mimo-io.cu:170-180). */
+int tomo_set_measured(tomo_ctx *ctx, const float *traces);
+
+/* Add seeded Gaussian noise of standard deviation rel * RMS(trace) to each
+ * receiver trace of float[nshots][nt][nr] in place.  No reference
+ * counterpart (the reference's noisy generators are unseeded). */
+int tomo_add_noise(float *traces, int nshots, int nt, int nr, double rel, uint64_t seed);
+
+/* Reference file layout: four arrays float[nshots][nt][nx] (bottom, right,
+ * top, left) with zeros off the sensor span, from/to float[nshots][nt][nr]
+ * (TOMO_RECV_LINES contexts only; generate-data.cu:187-223). */
+int tomo_traces_to_sides(const tomo_ctx *ctx, const float *traces, float *bottom,
+                         float *right, float *top, float *left);
+int tomo_sides_to_traces(const tomo_ctx *ctx, const float *bottom, const float *right,
+                         const float *top, const float *left, float *traces);
+
+/* -------------------------------------------------------------- gradient */
+
+/* For shots [g0, g0+ng): forward on the current field, residual
+ * measured - simulated at the receivers, time-reversed adjoint with residual
+ * injection, and the correlation G_g(i,j) = sum_k z(i,j,k) * Lu(i,j,k)
+ * (sirt-mimo-io.cu:333-353).  G_g is kept on the device per shot; misfit
+ * (may be NULL) receives sum of squared residuals of those shots (FP64). */
+int tomo_gradient(tomo_ctx *ctx, int g0, int ng, double *misfit);
+
+/* raw per-shot gradient G_g, float[ny][nx] */
+int tomo_get_gradient(tomo_ctx *ctx, int g, float *out);
+
+/* forward-wavefield energy sum_g sum_k u^2 of the last tomo_gradient() call
+ * (the sensor-coverage map), float[ny][nx]. */
+int tomo_get_coverage(tomo_ctx *ctx, float *out);
+
+/* ---------------------------------------------------------------- updates */
+
+/* f += scale * mask * (sum_g G_g) / (1 + f) [/ Ng]  (enum tomo_uniform_mode). */
+int tomo_update_uniform(tomo_ctx *ctx, float scale, int mode);
+
+/* Non-uniform weighted SIRT (sirt-mimo-norm.cu:56-118, 873-892):
+ * df_m = G_m / (1 + f); d_m = || df_m - mean_m df_m ||_2 over the full grid;
+ * f += scale * mask * sum_m (d_m^-1 / sum d^-1) df_m.  All reductions FP64.
+ * If some d_m is zero the weights fall back to uniform 1/M.
+ * weights_out (may be NULL) receives the normalised weights, double[Ng]. */
+int tomo_update_weighted(tomo_ctx *ctx, float scale, double *weights_out);
+
+/* ART step for one shot (art-mimo-io.cu:381-400, 684):
+ * f += scale * mask * G_g / (1 + f). */
+int tomo_update_group(tomo_ctx *ctx, int g, float scale);
+
+/* epsilon = 100 * ||f - fo|| / ||fo|| over the full grid, FP64
+ * (sirt-mimo.cu:342-344, 427-436). */
+int tomo_error(tomo_ctx *ctx, double *eps);
+
+/* ------------------------------------------------------------ iterations */
+
+/* One SIRT iteration over all shots (gradient of every shot in passes of
+ * shots_per_pass, then the update): weighted != 0 -> tomo_update_weighted,
+ * else tomo_update_uniform(mode).  misfit / eps may be NULL. */
+int tomo_sirt_iteration(tomo_ctx *ctx, float scale, int weighted, int mode,
+                        double *misfit, double *eps);
+
+/* One ART iteration: shots in order, gradient then update per shot
+ * (art-mimo-io.cu:637-685). */
+int tomo_art_iteration(tomo_ctx *ctx, float scale, double *misfit, double *eps);
+
+/* relaxation scale for 0-based iteration k (enum tomo_schedule_kind). */
+double tomo_schedule(int kind, int k, const double *params);
+
+/* -------------------------------------------------------------- text I/O */
+
+/* mimo-io.cuh:22-45 layout: for each l, k, j: values i = 0..nx-1 each
+ * followed by " ", then "\n"; "\n" after each k block and each l block.
+ * precision <= 0 -> max_digits10 (9).  trailing_blank_lines = 0 gives the
+ * IO_Files layout of sirt-mimo.cu:389-425 (one 2-D block, no blank lines). */
+int tomo_write_text(const char *path, const float *a, int nx, int ny, int nz, int nw,
+                    int precision, int trailing_blank_lines);
+/* whitespace-separated read of nx*ny*nz*nw values (mimo-io.cuh:6-20) */
+int tomo_read_text(const char *path, float *a, int nx, int ny, int nz, int nw);
+
+/* ----------------------------------------------------- instrumentation */
+
+/* Kernel timing gathered with HIP events on the context stream around the
+ * forward-step and adjoint-step loops since the last reset:
+ * out[0] = forward-step ms, out[1] = forward-step launches,
+ * out[2] = adjoint-step ms, out[3] = adjoint-step launches,
+ * out[4] = forward cell-updates, out[5] = adjoint cell-updates. */
+int tomo_timing(tomo_ctx *ctx, double *out6);
+int tomo_timing_reset(tomo_ctx *ctx);
+
+/* Tile shape of the stencil kernels (rows per thread); 0 = autotuned. */
+int tomo_set_tile(tomo_ctx *ctx, int rows_per_thread);
+int tomo_get_tile(const tomo_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMO_H */

@@ -1,0 +1,388 @@
+"""MI355X wave-tomography engine -- Python binding of ``libtomo_hip.so``.
+
+The compute path is the HIP library built from ``csrc/`` for gfx950; this
+module only marshals numpy arrays through the C ABI declared in
+``include/tomo.h``.  There is no CPU fallback: if the library is missing or
+no GPU is present, the engine calls raise.
+
+The reference drives the same loop from one ``Ultrasonic_Tomography()`` host
+function per executable (e.g. sirt-mimo-io.cu:97-417); :class:`Engine` exposes
+its steps one by one (set field / transducers / groups, forward, residual +
+adjoint + correlation, the three update rules, the error) and
+:func:`Engine.sirt_iteration` / :func:`Engine.art_iteration` run whole
+iterations.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libtomo_hip.so")
+
+# enum values of include/tomo.h
+WAVELET_REF_GAUSS_COS, WAVELET_RICKER = 0, 1
+RECV_LINES, RECV_POINTS = 0, 1
+UPDATE_SUM, UPDATE_MEAN = 0, 1
+SCHED_POWER, SCHED_ALTERNATING = 0, 1
+
+_STATUS = {0: "ok", -1: "bad argument", -2: "HIP error", -3: "allocation failed",
+           -4: "invalid state", -5: "I/O error", -6: "no GPU"}
+
+
+class TomoConfig(ctypes.Structure):
+    """Mirror of ``struct tomo_config`` (include/tomo.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nt", ctypes.c_int32),
+        ("wavelet", ctypes.c_int32),
+        ("h", ctypes.c_double), ("dt", ctypes.c_double), ("c0", ctypes.c_double),
+        ("wl_omega", ctypes.c_double), ("wl_tau", ctypes.c_double), ("wl_shift", ctypes.c_double),
+        ("ricker_fp", ctypes.c_double), ("ricker_t0", ctypes.c_double),
+        ("src_last_level", ctypes.c_int32), ("sponge_width", ctypes.c_int32),
+        ("sponge_gamma", ctypes.c_double),
+        ("recv_mode", ctypes.c_int32), ("record_first", ctypes.c_int32),
+        ("line_lo", ctypes.c_int32), ("line_hi", ctypes.c_int32),
+        ("line_first", ctypes.c_int32), ("line_last", ctypes.c_int32),
+        ("recv_gain", ctypes.c_double),
+        ("mask_x0", ctypes.c_int32), ("mask_x1", ctypes.c_int32),
+        ("mask_y0", ctypes.c_int32), ("mask_y1", ctypes.c_int32),
+        ("checkpoint_every", ctypes.c_int32), ("shots_per_pass", ctypes.c_int32),
+    ]
+
+    def copy(self) -> "TomoConfig":
+        c = TomoConfig()
+        ctypes.pointer(c)[0] = self
+        return c
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class TomoError(RuntimeError):
+    """Raised when a C-ABI call returns a negative status."""
+
+
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_int = ctypes.c_int
+
+# name -> (restype, argtypes) for every function of include/tomo.h
+SIGNATURES = {
+    "tomo_config_reference": (None, [_vp]),
+    "tomo_config_unit": (None, [_vp, _int, _int]),
+    "tomo_create": (_int, [_vp, _int, ctypes.POINTER(_vp)]),
+    "tomo_destroy": (None, [_vp]),
+    "tomo_last_error": (ctypes.c_char_p, [_vp]),
+    "tomo_strerror": (ctypes.c_char_p, [_int]),
+    "tomo_abi_version": (_int, []),
+    "tomo_transducers_reference": (_int, [_i32p, _i32p, _int]),
+    "tomo_transducers_ring": (_int, [_int, _int, _int, ctypes.c_double, _i32p, _i32p]),
+    "tomo_set_transducers": (_int, [_vp, _i32p, _i32p, _int, _int]),
+    "tomo_set_groups": (_int, [_vp, _int]),
+    "tomo_set_receivers": (_int, [_vp, _i32p, _i32p, _int, _int]),
+    "tomo_num_shots": (_int, [_vp]),
+    "tomo_num_receivers": (_int, [_vp]),
+    "tomo_get_receivers": (_int, [_vp, _i32p, _i32p]),
+    "tomo_set_field": (_int, [_vp, _f32p]),
+    "tomo_get_field": (_int, [_vp, _f32p]),
+    "tomo_set_truth": (_int, [_vp, _f32p]),
+    "tomo_phantom_reference": (_int, [_f32p, _int, _int, ctypes.c_double]),
+    "tomo_phantom_inclusions": (_int, [_f32p, _int, _int, ctypes.c_uint64, _int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       _int, _int, _int, _int]),
+    "tomo_phantom_smooth": (_int, [_f32p, _int, _int, ctypes.c_uint64, ctypes.c_double,
+                                   ctypes.c_double, _int, _int, _int, _int]),
+    "tomo_forward": (_int, [_vp, _int, _int, _int, _f32p]),
+    "tomo_set_measured": (_int, [_vp, _f32p]),
+    "tomo_add_noise": (_int, [_f32p, _int, _int, _int, ctypes.c_double, ctypes.c_uint64]),
+    "tomo_traces_to_sides": (_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "tomo_sides_to_traces": (_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
+    "tomo_gradient": (_int, [_vp, _int, _int, _vp]),
+    "tomo_get_gradient": (_int, [_vp, _int, _f32p]),
+    "tomo_get_coverage": (_int, [_vp, _f32p]),
+    "tomo_update_uniform": (_int, [_vp, ctypes.c_float, _int]),
+    "tomo_update_weighted": (_int, [_vp, ctypes.c_float, _vp]),
+    "tomo_update_group": (_int, [_vp, _int, ctypes.c_float]),
+    "tomo_error": (_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "tomo_sirt_iteration": (_int, [_vp, ctypes.c_float, _int, _int, _vp, _vp]),
+    "tomo_art_iteration": (_int, [_vp, ctypes.c_float, _vp, _vp]),
+    "tomo_schedule": (ctypes.c_double, [_int, _int, _f64p]),
+    "tomo_write_text": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int, _int, _int, _int]),
+    "tomo_read_text": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int, _int]),
+    "tomo_timing": (_int, [_vp, _f64p]),
+    "tomo_timing_reset": (_int, [_vp]),
+    "tomo_set_tile": (_int, [_vp, _int]),
+    "tomo_get_tile": (_int, [_vp]),
+}
+
+
+def lib():
+    """Load libtomo_hip.so (built by ``__graft_entry__.build()``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TomoError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status, ctx=None):
+    if status < 0:
+        msg = _STATUS.get(status, str(status))
+        if ctx is not None:
+            detail = lib().tomo_last_error(ctx)
+            if detail:
+                msg += ": " + detail.decode(errors="replace")
+        raise TomoError(msg)
+    return status
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+# --------------------------------------------------------------- presets
+
+def config_reference() -> TomoConfig:
+    c = TomoConfig()
+    lib().tomo_config_reference(ctypes.byref(c))
+    return c
+
+
+def config_unit(n: int, nt: int) -> TomoConfig:
+    c = TomoConfig()
+    lib().tomo_config_unit(ctypes.byref(c), int(n), int(nt))
+    return c
+
+
+def transducers_reference(ns: int = 640):
+    ii = np.zeros(ns, np.int32)
+    jj = np.zeros(ns, np.int32)
+    _check(lib().tomo_transducers_reference(ii, jj, ns))
+    return ii, jj
+
+
+def transducers_ring(nx: int, ny: int, count: int, radius: float):
+    ii = np.zeros(count, np.int32)
+    jj = np.zeros(count, np.int32)
+    _check(lib().tomo_transducers_ring(nx, ny, count, float(radius), ii, jj))
+    return ii, jj
+
+
+def phantom_reference(nx: int = 192, ny: int = 192, h: float = float(np.float32(0.001))):
+    fo = np.zeros(nx * ny, np.float32)
+    _check(lib().tomo_phantom_reference(fo, nx, ny, h))
+    return fo
+
+
+def phantom_inclusions(fo, nx, ny, seed, count, rmin, rmax, cmin, cmax, region):
+    x0, x1, y0, y1 = region
+    _check(lib().tomo_phantom_inclusions(fo, nx, ny, seed, count, rmin, rmax, cmin, cmax, x0, x1, y0, y1))
+    return fo
+
+
+def phantom_smooth(fo, nx, ny, seed, sigma, amp, region):
+    x0, x1, y0, y1 = region
+    _check(lib().tomo_phantom_smooth(fo, nx, ny, seed, sigma, amp, x0, x1, y0, y1))
+    return fo
+
+
+def add_noise(traces, rel, seed):
+    t = _f32(traces)
+    S, nt, nr = t.shape
+    _check(lib().tomo_add_noise(t, S, nt, nr, float(rel), int(seed)))
+    return t
+
+
+def schedule(kind: int, k: int, params) -> float:
+    return float(lib().tomo_schedule(int(kind), int(k), np.ascontiguousarray(params, np.float64)))
+
+
+def write_text(path, a, dims, precision=0, trailing_blank_lines=1):
+    nx, ny, nz, nw = (list(dims) + [1, 1, 1])[:4]
+    _check(lib().tomo_write_text(os.fsencode(path), _f32(a).ravel(), nx, ny, nz, nw, precision,
+                                 trailing_blank_lines))
+
+
+def read_text(path, dims):
+    nx, ny, nz, nw = (list(dims) + [1, 1, 1])[:4]
+    a = np.zeros(nx * ny * nz * nw, np.float32)
+    _check(lib().tomo_read_text(os.fsencode(path), a, nx, ny, nz, nw))
+    return a
+
+
+# ---------------------------------------------------------------- engine
+
+class Engine:
+    """One engine context (one HIP stream on one GPU)."""
+
+    def __init__(self, cfg: TomoConfig, device: int = -1):
+        self.cfg = cfg.copy()
+        self._ctx = _vp()
+        _check(lib().tomo_create(ctypes.byref(self.cfg), int(device), ctypes.byref(self._ctx)))
+        self.n2 = cfg.nx * cfg.ny
+
+    def close(self):
+        if self._ctx:
+            lib().tomo_destroy(self._ctx)
+            self._ctx = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, status):
+        return _check(status, self._ctx)
+
+    # geometry
+    def set_transducers(self, ii, jj, one_based=True):
+        self._c(lib().tomo_set_transducers(self._ctx, _i32(ii), _i32(jj), len(ii), int(one_based)))
+
+    def set_groups(self, group_size: int) -> int:
+        return self._c(lib().tomo_set_groups(self._ctx, int(group_size)))
+
+    def set_receivers(self, ri, rj, one_based=False):
+        self._c(lib().tomo_set_receivers(self._ctx, _i32(ri), _i32(rj), len(ri), int(one_based)))
+
+    @property
+    def num_shots(self) -> int:
+        return self._c(lib().tomo_num_shots(self._ctx))
+
+    @property
+    def num_receivers(self) -> int:
+        return self._c(lib().tomo_num_receivers(self._ctx))
+
+    def receivers(self):
+        n = self.num_receivers
+        ri = np.zeros(n, np.int32)
+        rj = np.zeros(n, np.int32)
+        self._c(lib().tomo_get_receivers(self._ctx, ri, rj))
+        return ri, rj
+
+    # fields
+    def set_field(self, f):
+        self._c(lib().tomo_set_field(self._ctx, _f32(f).ravel()))
+
+    def get_field(self):
+        f = np.zeros(self.n2, np.float32)
+        self._c(lib().tomo_get_field(self._ctx, f))
+        return f
+
+    def set_truth(self, fo):
+        self._c(lib().tomo_set_truth(self._ctx, _f32(fo).ravel()))
+
+    # data
+    def forward(self, g0=0, ng=None, fp64=False):
+        if ng is None:
+            ng = self.num_shots - g0
+        tr = np.zeros((ng, self.cfg.nt, self.num_receivers), np.float32)
+        self._c(lib().tomo_forward(self._ctx, int(g0), int(ng), int(bool(fp64)), tr))
+        return tr
+
+    def set_measured(self, traces):
+        self._c(lib().tomo_set_measured(self._ctx, _f32(traces)))
+
+    def traces_to_sides(self, traces):
+        S = traces.shape[0]
+        sides = [np.zeros((S, self.cfg.nt, self.cfg.nx), np.float32) for _ in range(4)]
+        self._c(lib().tomo_traces_to_sides(self._ctx, _f32(traces), *sides))
+        return sides
+
+    def sides_to_traces(self, bottom, right, top, left):
+        S = bottom.shape[0]
+        tr = np.zeros((S, self.cfg.nt, self.num_receivers), np.float32)
+        self._c(lib().tomo_sides_to_traces(self._ctx, _f32(bottom), _f32(right), _f32(top), _f32(left), tr))
+        return tr
+
+    # gradient
+    def gradient(self, g0=0, ng=None) -> float:
+        if ng is None:
+            ng = self.num_shots - g0
+        mis = ctypes.c_double(0.0)
+        self._c(lib().tomo_gradient(self._ctx, int(g0), int(ng), ctypes.cast(ctypes.byref(mis), _vp)))
+        return mis.value
+
+    def get_gradient(self, g):
+        out = np.zeros(self.n2, np.float32)
+        self._c(lib().tomo_get_gradient(self._ctx, int(g), out))
+        return out
+
+    def get_coverage(self):
+        out = np.zeros(self.n2, np.float32)
+        self._c(lib().tomo_get_coverage(self._ctx, out))
+        return out
+
+    # updates
+    def update_uniform(self, scale, mode=UPDATE_MEAN):
+        self._c(lib().tomo_update_uniform(self._ctx, float(scale), int(mode)))
+
+    def update_weighted(self, scale):
+        w = np.zeros(self.num_shots, np.float64)
+        self._c(lib().tomo_update_weighted(self._ctx, float(scale), _ptr(w)))
+        return w
+
+    def update_group(self, g, scale):
+        self._c(lib().tomo_update_group(self._ctx, int(g), float(scale)))
+
+    def error(self) -> float:
+        e = ctypes.c_double(0.0)
+        self._c(lib().tomo_error(self._ctx, ctypes.byref(e)))
+        return e.value
+
+    def sirt_iteration(self, scale, weighted=False, mode=UPDATE_MEAN):
+        mis = ctypes.c_double(0.0)
+        eps = ctypes.c_double(0.0)
+        self._c(lib().tomo_sirt_iteration(self._ctx, float(scale), int(bool(weighted)), int(mode),
+                                          ctypes.cast(ctypes.byref(mis), _vp),
+                                          ctypes.cast(ctypes.byref(eps), _vp)))
+        return mis.value, eps.value
+
+    def art_iteration(self, scale):
+        mis = ctypes.c_double(0.0)
+        eps = ctypes.c_double(0.0)
+        self._c(lib().tomo_art_iteration(self._ctx, float(scale), ctypes.cast(ctypes.byref(mis), _vp),
+                                         ctypes.cast(ctypes.byref(eps), _vp)))
+        return mis.value, eps.value
+
+    # instrumentation
+    def timing(self) -> dict:
+        out = np.zeros(6, np.float64)
+        self._c(lib().tomo_timing(self._ctx, out))
+        return {"fwd_ms": out[0], "fwd_launches": int(out[1]), "adj_ms": out[2],
+                "adj_launches": int(out[3]), "fwd_cells": out[4], "adj_cells": out[5]}
+
+    def timing_reset(self):
+        self._c(lib().tomo_timing_reset(self._ctx))
+
+    def set_tile(self, rows_per_thread: int):
+        self._c(lib().tomo_set_tile(self._ctx, int(rows_per_thread)))
+
+    def get_tile(self) -> int:
+        return self._c(lib().tomo_get_tile(self._ctx))
