@@ -179,17 +179,18 @@ int tomo_set_truth(tomo_ctx *ctx, const float *fo);
  * generate-field.cu:117-172).  x = -0.1 + i * h. */
 int tomo_phantom_reference(float *fo, int nx, int ny, double h);
 
-/* Seeded circular inclusions in a zero background inside the region
- * [x0,x1) x [y0,y1): count disks, radius uniform in [rmin, rmax] cells,
- * velocity contrast uniform in +-[cmin, cmax]; f = (1 + contrast)^2 - 1.
- * Added on top of the existing values of fo.  No reference counterpart. */
+/* Seeded circular inclusions inside the region [x0,x1) x [y0,y1): count
+ * disks, radius uniform in [rmin, rmax] cells, velocity contrast uniform in
+ * +-[cmin, cmax]; every cell of a disk is set to f = (1 + contrast)^2 - 1,
+ * other cells keep their value.  No reference counterpart. */
 int tomo_phantom_inclusions(float *fo, int nx, int ny, uint64_t seed, int count,
                             double rmin, double rmax, double cmin, double cmax,
                             int x0, int x1, int y0, int y1);
 
 /* Seeded smooth random velocity: white noise filtered by a Gaussian of
- * width sigma cells, scaled to a peak contrast of +-amp, f = (1+c)^2 - 1,
- * confined to [x0,x1) x [y0,y1).  No reference counterpart. */
+ * width sigma cells, scaled to a peak contrast of +-amp and tapered to zero
+ * over sigma cells at the edge of [x0,x1) x [y0,y1); sets every cell of fo
+ * to f = (1 + c)^2 - 1 (0 outside the region).  No reference counterpart. */
 int tomo_phantom_smooth(float *fo, int nx, int ny, uint64_t seed, double sigma, double amp,
                         int x0, int x1, int y0, int y1);
 
@@ -229,8 +230,11 @@ int tomo_gradient(tomo_ctx *ctx, int g0, int ng, double *misfit);
 /* raw per-shot gradient G_g, float[ny][nx] */
 int tomo_get_gradient(tomo_ctx *ctx, int g, float *out);
 
-/* forward-wavefield energy sum_g sum_k u^2 of the last tomo_gradient() call
- * (the sensor-coverage map), float[ny][nx]. */
+/* Sensor-coverage map: forward-wavefield energy sum_g sum_k u(k)^2 over the
+ * adjoint sweep (k = 1..nt-2), accumulated by tomo_gradient() while enabled
+ * (off by default; it adds one read-modify-write per cell update).
+ * float[ny][nx].  No reference counterpart. */
+int tomo_enable_coverage(tomo_ctx *ctx, int on);
 int tomo_get_coverage(tomo_ctx *ctx, float *out);
 
 /* ---------------------------------------------------------------- updates */

@@ -105,6 +105,7 @@ SIGNATURES = {
     "tomo_sides_to_traces": (_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
     "tomo_gradient": (_int, [_vp, _int, _int, _vp]),
     "tomo_get_gradient": (_int, [_vp, _int, _f32p]),
+    "tomo_enable_coverage": (_int, [_vp, _int]),
     "tomo_get_coverage": (_int, [_vp, _f32p]),
     "tomo_update_uniform": (_int, [_vp, ctypes.c_float, _int]),
     "tomo_update_weighted": (_int, [_vp, ctypes.c_float, _vp]),
@@ -333,6 +334,9 @@ class Engine:
         out = np.zeros(self.n2, np.float32)
         self._c(lib().tomo_get_gradient(self._ctx, int(g), out))
         return out
+
+    def enable_coverage(self, on=True):
+        self._c(lib().tomo_enable_coverage(self._ctx, int(bool(on))))
 
     def get_coverage(self):
         out = np.zeros(self.n2, np.float32)

@@ -1,0 +1,942 @@
+// tomo_engine.hip -- context, memory plan and the engine half of the C ABI
+// (include/tomo.h).  Replaces the per-executable Ultrasonic_Tomography()
+// host loops of the reference (e.g. sirt-mimo-io.cu:97-417,
+// sirt-mimo-norm-mem.cu:419-498, art-mimo-io.cu:631-718).
+//
+// Memory plan for a pass of P shots (all float, [level][P][ny][nx]):
+//   full history (checkpoint_every == 0 or >= nt): nt levels, one forward;
+//   checkpointing (C): level pairs (mC, mC+1) of every segment m, plus a
+//     C-level segment buffer; the backward sweep recomputes each segment
+//     from its pair, newest segment first, then runs the adjoint over it.
+//   adjoint: a ring of 3 levels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tomo.h"
+#include "tomo_host.h"
+#include "tomo_kernels.h"
+
+using namespace tomo;
+
+struct TimedSpan {
+  hipEvent_t a, b;
+  int kind;          // 0 forward, 1 adjoint
+  double launches;
+  double cells;
+};
+
+struct tomo_ctx {
+  tomo_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nx = 0, ny = 0, nt = 0;
+  long long n2 = 0;
+  // geometry (host)
+  std::vector<int> tx, ty;  // transducers, 0-based
+  std::vector<int4> bbox;   // per shot
+  std::vector<int> rcell, ranchor;
+  int nshots = 0, nr = 0;
+  // persistent device buffers
+  float *d_f = nullptr, *d_fo = nullptr, *d_pw = nullptr;
+  double *d_pw64 = nullptr;
+  bool pw64_valid = false;
+  int *d_cellmap = nullptr, *d_anchor = nullptr;
+  int4 *d_bbox = nullptr;
+  float *d_spa = nullptr, *d_spab = nullptr;
+  double *d_spa64 = nullptr, *d_spab64 = nullptr;
+  float *d_meas = nullptr, *d_rr = nullptr, *d_grad = nullptr, *d_cov = nullptr;
+  double *d_misfit = nullptr;
+  double *d_scratch = nullptr;
+  size_t scratch_n = 0;
+  // wavefield pool
+  float *d_ckpt = nullptr, *d_seg = nullptr, *d_z = nullptr;
+  size_t ckpt_n = 0, seg_n = 0, z_n = 0;
+  double *d_ring64 = nullptr;
+  size_t ring64_n = 0;
+  // state
+  bool have_truth = false, have_meas = false, coverage = false;
+  std::vector<char> grad_valid;
+  std::vector<float> amp32;   // pulse per step, float arithmetic
+  std::vector<double> amp64;  // pulse per step, double arithmetic
+  int tile_ry = 0;
+  // timing
+  std::vector<hipEvent_t> ev_free;
+  std::vector<TimedSpan> spans;
+  double t_ms[2] = {0, 0}, t_launch[2] = {0, 0}, t_cells[2] = {0, 0};
+};
+
+namespace {
+
+int fail(tomo_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail((ctx), TOMO_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(tomo_ctx *c, T **p, size_t n) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) return TOMO_OK;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, TOMO_ERR_ALLOC, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  return TOMO_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+int full_history(const tomo_ctx *c) {
+  const int C = c->cfg.checkpoint_every;
+  return C <= 0 || C >= c->nt;
+}
+
+int pass_shots(const tomo_ctx *c) {
+  const int P = c->cfg.shots_per_pass;
+  return (P > 0 && P < c->nshots) ? P : std::max(c->nshots, 1);
+}
+
+// rows per thread of the stencil tiles
+int tile_rows(const tomo_ctx *c) {
+  if (c->tile_ry) return c->tile_ry;
+  // taller strips amortise the strip halo; short grids keep enough blocks
+  return c->ny >= 256 ? 16 : 8;
+}
+
+hipEvent_t take_event(tomo_ctx *c) {
+  if (!c->ev_free.empty()) {
+    hipEvent_t e = c->ev_free.back();
+    c->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void span_begin(tomo_ctx *c, TimedSpan &s, int kind) {
+  s.a = take_event(c);
+  s.b = take_event(c);
+  s.kind = kind;
+  s.launches = 0;
+  s.cells = 0;
+  if (s.a) hipEventRecord(s.a, c->stream);
+}
+
+void span_end(tomo_ctx *c, TimedSpan &s) {
+  if (s.b) hipEventRecord(s.b, c->stream);
+  c->spans.push_back(s);
+}
+
+// after a stream sync: fold recorded spans into the totals
+void spans_collect(tomo_ctx *c) {
+  for (auto &s : c->spans) {
+    float ms = 0.f;
+    if (s.a && s.b && hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+      c->t_ms[s.kind] += ms;
+      c->t_launch[s.kind] += s.launches;
+      c->t_cells[s.kind] += s.cells;
+    }
+    if (s.a) c->ev_free.push_back(s.a);
+    if (s.b) c->ev_free.push_back(s.b);
+  }
+  c->spans.clear();
+}
+
+int upload_coefficients(tomo_ctx *c) {
+  const double c0dt_h = c->cfg.c0 * c->cfg.dt / c->cfg.h;
+  HIPCHK(c, launch_coefficients<float>(c->d_f, c->d_pw, c->nx, c->ny, c0dt_h, c->cfg.sponge_width, c->d_spa,
+                                       c->stream));
+  c->pw64_valid = false;
+  return TOMO_OK;
+}
+
+int ensure_pw64(tomo_ctx *c) {
+  if (c->pw64_valid) return TOMO_OK;
+  if (!c->d_pw64) {
+    int rc = dalloc(c, &c->d_pw64, (size_t)c->n2);
+    if (rc) return rc;
+  }
+  const double c0dt_h = c->cfg.c0 * c->cfg.dt / c->cfg.h;
+  HIPCHK(c, launch_coefficients<double>(c->d_f, c->d_pw64, c->nx, c->ny, c0dt_h, c->cfg.sponge_width,
+                                        c->d_spa64, c->stream));
+  c->pw64_valid = true;
+  return TOMO_OK;
+}
+
+int validate_config(const tomo_config &g, std::string &why) {
+  if (g.nx < 8 || g.ny < 8) return why = "grid must be at least 8 x 8", TOMO_ERR_ARG;
+  if ((long long)g.nx * g.ny > (1LL << 30)) return why = "grid too large", TOMO_ERR_ARG;
+  if (g.nt < 3) return why = "nt must be >= 3", TOMO_ERR_ARG;
+  if (!(g.h > 0) || !(g.dt > 0) || !(g.c0 > 0)) return why = "h, dt, c0 must be positive", TOMO_ERR_ARG;
+  if (g.wavelet != TOMO_WAVELET_REF_GAUSS_COS && g.wavelet != TOMO_WAVELET_RICKER)
+    return why = "unknown wavelet", TOMO_ERR_ARG;
+  if (g.sponge_width < 0 || 2 * g.sponge_width >= std::min(g.nx, g.ny) || g.sponge_width > 256)
+    return why = "sponge width out of range", TOMO_ERR_ARG;
+  if (g.recv_mode != TOMO_RECV_LINES && g.recv_mode != TOMO_RECV_POINTS)
+    return why = "unknown receiver mode", TOMO_ERR_ARG;
+  if (g.recv_mode == TOMO_RECV_LINES) {
+    const int n = std::min(g.nx, g.ny);
+    if (g.line_lo < 2 || g.line_hi > n - 3 || g.line_lo + 2 > g.line_hi)
+      return why = "sensor lines must lie >= 2 cells inside the edge", TOMO_ERR_ARG;
+    if (g.line_first < 2 || g.line_last > n - 3 || g.line_first > g.line_last ||
+        (g.line_first <= g.line_lo && g.line_last >= g.line_lo) ||
+        (g.line_first <= g.line_hi && g.line_last >= g.line_hi))
+      return why = "sensor span must avoid the line rows", TOMO_ERR_ARG;
+  }
+  if (g.mask_x0 < 0 || g.mask_y0 < 0 || g.mask_x1 > g.nx || g.mask_y1 > g.ny || g.mask_x0 > g.mask_x1 ||
+      g.mask_y0 > g.mask_y1)
+    return why = "mask outside the grid", TOMO_ERR_ARG;
+  if (g.checkpoint_every > 0 && g.checkpoint_every < 3) return why = "checkpoint_every must be 0 or >= 3", TOMO_ERR_ARG;
+  if (g.shots_per_pass < 0) return why = "shots_per_pass must be >= 0", TOMO_ERR_ARG;
+  if (g.record_first < 0) return why = "record_first must be >= 0", TOMO_ERR_ARG;
+  return TOMO_OK;
+}
+
+int upload_receivers(tomo_ctx *c) {
+  std::vector<int> map((size_t)c->n2, -1);
+  for (size_t r = 0; r < c->rcell.size(); ++r) map[c->rcell[r]] = (int)r;
+  HIPCHK(c, hipMemcpyAsync(c->d_cellmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, c->stream));
+  int rc = dalloc(c, &c->d_anchor, std::max<size_t>(c->ranchor.size(), 1));
+  if (rc) return rc;
+  if (!c->ranchor.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_anchor, c->ranchor.data(), sizeof(int) * c->ranchor.size(),
+                             hipMemcpyHostToDevice, c->stream));
+  c->nr = (int)c->rcell.size();
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+// (re)allocate per-shot buffers after the shot or receiver count changed
+int alloc_shot_buffers(tomo_ctx *c) {
+  const size_t S = (size_t)std::max(c->nshots, 1), tr = (size_t)c->nt * std::max(c->nr, 1);
+  int rc;
+  if ((rc = dalloc(c, &c->d_meas, S * tr))) return rc;
+  if ((rc = dalloc(c, &c->d_rr, S * tr))) return rc;
+  if ((rc = dalloc(c, &c->d_grad, S * (size_t)c->n2))) return rc;
+  if ((rc = dalloc(c, &c->d_misfit, S))) return rc;
+  if (c->coverage && (rc = dalloc(c, &c->d_cov, S * (size_t)c->n2))) return rc;
+  HIPCHK(c, launch_fill_f32(c->d_rr, S * tr, 0.f, c->stream));
+  HIPCHK(c, launch_fill_f32(c->d_grad, S * (size_t)c->n2, 0.f, c->stream));
+  c->have_meas = false;
+  c->grad_valid.assign(S, 0);
+  return TOMO_OK;
+}
+
+int ensure_scratch(tomo_ctx *c, size_t n) {
+  if (c->scratch_n >= n) return TOMO_OK;
+  int rc = dalloc(c, &c->d_scratch, n);
+  if (rc) return rc;
+  c->scratch_n = n;
+  return TOMO_OK;
+}
+
+// wavefield storage for passes of P shots
+int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
+  const size_t lvl = (size_t)P * c->n2;
+  size_t seg_levels, ckpt_levels;
+  if (!need_history) {
+    seg_levels = 3;
+    ckpt_levels = 0;
+  } else if (full_history(c)) {
+    seg_levels = (size_t)c->nt;
+    ckpt_levels = 0;
+  } else {
+    const int C = c->cfg.checkpoint_every;
+    seg_levels = (size_t)C;
+    ckpt_levels = 2 * (size_t)((c->nt + C - 1) / C);
+  }
+  int rc;
+  if (c->seg_n < seg_levels * lvl) {
+    if ((rc = dalloc(c, &c->d_seg, seg_levels * lvl))) return rc;
+    c->seg_n = seg_levels * lvl;
+  }
+  if (c->ckpt_n < ckpt_levels * lvl) {
+    if ((rc = dalloc(c, &c->d_ckpt, ckpt_levels * lvl))) return rc;
+    c->ckpt_n = ckpt_levels * lvl;
+  }
+  if (need_history && c->z_n < 3 * lvl) {
+    if ((rc = dalloc(c, &c->d_z, 3 * lvl))) return rc;
+    c->z_n = 3 * lvl;
+  }
+  return TOMO_OK;
+}
+
+// location of forward level k during the first (recording) pass
+float *pass1_level(tomo_ctx *c, int P, int k, bool need_history) {
+  const size_t lvl = (size_t)P * c->n2;
+  if (!need_history) return c->d_seg + (size_t)(k % 3) * lvl;
+  if (full_history(c)) return c->d_seg + (size_t)k * lvl;
+  const int C = c->cfg.checkpoint_every;
+  const int q = k % C;
+  if (q < 2) return c->d_ckpt + ((size_t)(k / C) * 2 + q) * lvl;
+  return c->d_seg + (size_t)(k % 3) * lvl;
+}
+
+// location of level k while segment m is being replayed
+float *segment_level(tomo_ctx *c, int P, int m, int k) {
+  const size_t lvl = (size_t)P * c->n2;
+  if (full_history(c)) return c->d_seg + (size_t)k * lvl;
+  const int C = c->cfg.checkpoint_every;
+  const int q = k - m * C;
+  if (q < 2) return c->d_ckpt + ((size_t)m * 2 + q) * lvl;
+  return c->d_seg + (size_t)q * lvl;
+}
+
+template <typename T>
+FwdArgs<T> fwd_args(tomo_ctx *c, int shot0, const T *pw, const T *spa, const T *spab) {
+  FwdArgs<T> a{};
+  a.sstride = c->n2;
+  a.pw = pw;
+  a.f = c->d_f;
+  a.cellmap = c->d_cellmap;
+  a.bbox = c->d_bbox;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nt = c->nt;
+  a.nr = c->nr;
+  a.shot0 = shot0;
+  a.c0dt_h = T(c->cfg.c0 * c->cfg.dt / c->cfg.h);
+  a.sponge_w = c->cfg.sponge_width;
+  a.sp_a = spa;
+  a.sp_ab = spab;
+  return a;
+}
+
+template <typename T>
+T pulse_h2(const tomo_ctx *c, int k);
+template <>
+float pulse_h2<float>(const tomo_ctx *c, int k) {
+  return c->amp32[k] * (float)(c->cfg.h * c->cfg.h);
+}
+template <>
+double pulse_h2<double>(const tomo_ctx *c, int k) {
+  return c->amp64[k] * (c->cfg.h * c->cfg.h);
+}
+
+// One forward step k (levels k-1, k -> k+1) for P shots from shot0.
+template <typename T>
+int forward_step(tomo_ctx *c, FwdArgs<T> a, const T *um, const T *u0, T *up, int k, int P, bool record,
+                 const float *meas, float *rec) {
+  a.um = um;
+  a.u0 = u0;
+  a.up = up;
+  a.level = k + 1;
+  a.record = record && (k + 1 >= c->cfg.record_first) && rec != nullptr && c->nr > 0;
+  a.meas = meas;
+  a.rec = rec;
+  a.amp_h2 = pulse_h2<T>(c, k);
+  HIPCHK(c, launch_forward_step<T>(a, P, tile_rows(c), c->stream));
+  return TOMO_OK;
+}
+
+// traces of shots [g0, g0+ng) into d_rr (as u values), float or double solver
+template <typename T>
+int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const T *pw, const T *spa,
+                   const T *spab) {
+  const int P = std::min(pass_shots(c), ng);
+  const size_t tr = (size_t)c->nt * c->nr;
+  for (int p0 = g0; p0 < g0 + ng; p0 += P) {
+    const int np = std::min(P, g0 + ng - p0);
+    HIPCHK(c, launch_fill_f32(c->d_rr + (size_t)p0 * tr, (size_t)np * tr, 0.f, c->stream));
+    auto L = [&](int k) { return ring + (size_t)(k % 3) * ring_lvl; };
+    if (std::is_same<T, float>::value) {
+      HIPCHK(c, launch_fill_f32((float *)L(0), (size_t)np * c->n2, 0.f, c->stream));
+      HIPCHK(c, launch_fill_f32((float *)L(1), (size_t)np * c->n2, 0.f, c->stream));
+    } else {
+      HIPCHK(c, launch_fill_f64((double *)L(0), (size_t)np * c->n2, 0.0, c->stream));
+      HIPCHK(c, launch_fill_f64((double *)L(1), (size_t)np * c->n2, 0.0, c->stream));
+    }
+    FwdArgs<T> a = fwd_args<T>(c, p0, pw, spa, spab);
+    TimedSpan s;
+    span_begin(c, s, 0);
+    for (int k = 1; k < c->nt - 1; ++k) {
+      int rc = forward_step<T>(c, a, L(k - 1), L(k), L(k + 1), k, np, true, nullptr, c->d_rr);
+      if (rc) return rc;
+    }
+    s.launches = c->nt - 2;
+    s.cells = (double)(c->nt - 2) * np * c->n2;
+    span_end(c, s);
+  }
+  return TOMO_OK;
+}
+
+AdjArgs adj_args(tomo_ctx *c, int shot0) {
+  AdjArgs a{};
+  a.sstride = c->n2;
+  a.pw = c->d_pw;
+  a.cellmap = c->d_cellmap;
+  a.anchor = c->d_anchor;
+  a.rr = c->d_rr;
+  a.acc = c->d_grad;
+  a.cov = c->coverage ? c->d_cov : nullptr;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nt = c->nt;
+  a.nr = c->nr;
+  a.shot0 = shot0;
+  a.gain = (float)c->cfg.recv_gain;
+  a.inv_h2 = (float)(1.0 / (c->cfg.h * c->cfg.h));
+  a.sponge_w = c->cfg.sponge_width;
+  a.sp_a = c->d_spa;
+  a.sp_ab = c->d_spab;
+  return a;
+}
+
+// Gradient of shots [p0, p0+np) in one pass (np <= P).
+int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
+  const size_t tr = (size_t)c->nt * c->nr;
+  const size_t lvl = (size_t)P * c->n2;
+  const int nt = c->nt;
+  HIPCHK(c, launch_fill_f32(c->d_rr + (size_t)p0 * tr, (size_t)np * tr, 0.f, c->stream));
+  // pass 1: forward with the residual meas - u recorded at the receivers
+  HIPCHK(c, launch_fill_f32(pass1_level(c, P, 0, true), (size_t)np * c->n2, 0.f, c->stream));
+  HIPCHK(c, launch_fill_f32(pass1_level(c, P, 1, true), (size_t)np * c->n2, 0.f, c->stream));
+  FwdArgs<float> fa = fwd_args<float>(c, p0, c->d_pw, c->d_spa, c->d_spab);
+  {
+    TimedSpan s;
+    span_begin(c, s, 0);
+    for (int k = 1; k < nt - 1; ++k) {
+      int rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
+                                   pass1_level(c, P, k + 1, true), k, np, true, c->d_meas, c->d_rr);
+      if (rc) return rc;
+    }
+    s.launches = nt - 2;
+    s.cells = (double)(nt - 2) * np * c->n2;
+    span_end(c, s);
+  }
+  HIPCHK(c, launch_misfit(c->d_rr + (size_t)p0 * tr, (long long)tr, np, c->d_misfit + p0, c->stream));
+  // pass 2: adjoint sweep, newest segment first
+  HIPCHK(c, launch_fill_f32(c->d_grad + (size_t)p0 * c->n2, (size_t)np * c->n2, 0.f, c->stream));
+  if (c->coverage) HIPCHK(c, launch_fill_f32(c->d_cov + (size_t)p0 * c->n2, (size_t)np * c->n2, 0.f, c->stream));
+  auto Z = [&](int k) { return c->d_z + (size_t)(k % 3) * lvl; };
+  HIPCHK(c, launch_fill_f32(Z(nt - 1), (size_t)np * c->n2, 0.f, c->stream));
+  HIPCHK(c, launch_fill_f32(Z(nt), (size_t)np * c->n2, 0.f, c->stream));
+  const bool full = full_history(c);
+  const int C = full ? nt : c->cfg.checkpoint_every;
+  const int nseg = (nt + C - 1) / C;
+  AdjArgs aa = adj_args(c, p0);
+  for (int m = nseg - 1; m >= 0; --m) {
+    const int k0 = m * C;
+    const int cnt = std::min(C, nt - k0);
+    if (!full && cnt > 2) {
+      TimedSpan s;
+      span_begin(c, s, 0);
+      for (int q = 2; q < cnt; ++q) {
+        const int k = k0 + q - 1;
+        int rc = forward_step<float>(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
+                                     segment_level(c, P, m, k + 1), k, np, false, nullptr, nullptr);
+        if (rc) return rc;
+      }
+      s.launches = cnt - 2;
+      s.cells = (double)(cnt - 2) * np * c->n2;
+      span_end(c, s);
+    }
+    TimedSpan s;
+    span_begin(c, s, 1);
+    int launches = 0;
+    for (int q = cnt - 1; q >= 0; --q) {
+      const int k = k0 + q;
+      if (k < 1 || k > nt - 2) continue;
+      aa.z1 = Z(k + 1);
+      aa.z2 = Z(k + 2);
+      aa.z0 = Z(k);
+      aa.u = segment_level(c, P, m, k);
+      aa.level = k;
+      HIPCHK(c, launch_adjoint_step(aa, np, tile_rows(c), c->stream));
+      ++launches;
+    }
+    s.launches = launches;
+    s.cells = (double)launches * np * c->n2;
+    span_end(c, s);
+  }
+  return TOMO_OK;
+}
+
+int check_ctx(tomo_ctx *c) {
+  if (!c) return TOMO_ERR_ARG;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return fail(c, TOMO_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  return TOMO_OK;
+}
+
+int need_shots(tomo_ctx *c) {
+  if (c->nshots <= 0) return fail(c, TOMO_ERR_STATE, "no source groups: call tomo_set_groups first");
+  if (c->nr <= 0) return fail(c, TOMO_ERR_STATE, "no receivers");
+  return TOMO_OK;
+}
+
+Mask ctx_mask(const tomo_ctx *c) { return Mask{c->cfg.mask_x0, c->cfg.mask_x1, c->cfg.mask_y0, c->cfg.mask_y1}; }
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int tomo_create(const tomo_config *cfg, int device, tomo_ctx **out) {
+  if (!cfg || !out) return TOMO_ERR_ARG;
+  *out = nullptr;
+  std::string why;
+  int rc = validate_config(*cfg, why);
+  if (rc) {
+    tomo_host::set_create_error(why);
+    return rc;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    tomo_host::set_create_error("no HIP device");
+    return TOMO_ERR_NODEV;
+  }
+  if (device < 0) {
+    if (hipGetDevice(&device) != hipSuccess) device = 0;
+  }
+  if (device >= ndev) {
+    tomo_host::set_create_error("device index out of range");
+    return TOMO_ERR_ARG;
+  }
+  tomo_ctx *c = new tomo_ctx();
+  c->cfg = *cfg;
+  c->device = device;
+  c->nx = cfg->nx;
+  c->ny = cfg->ny;
+  c->nt = cfg->nt;
+  c->n2 = (long long)cfg->nx * cfg->ny;
+  auto bail = [&](int code) {
+    tomo_host::set_create_error(c->err);
+    tomo_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(c, TOMO_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(c, TOMO_ERR_HIP, "hipStreamCreate failed"));
+  if ((rc = dalloc(c, &c->d_f, c->n2)) || (rc = dalloc(c, &c->d_fo, c->n2)) || (rc = dalloc(c, &c->d_pw, c->n2)) ||
+      (rc = dalloc(c, &c->d_cellmap, c->n2)) || (rc = dalloc(c, &c->d_bbox, 1)))
+    return bail(rc);
+  // sponge tables (also allocated, 1 entry, when there is no sponge)
+  const int W = std::max(cfg->sponge_width, 1);
+  std::vector<double> a64(W, 1.0), ab64(W, 1.0);
+  std::vector<float> a32(W, 1.f), ab32(W, 1.f);
+  for (int d = 0; d < cfg->sponge_width; ++d) {
+    const double g = tomo_host::sponge_gamma(*cfg, d);
+    a64[d] = 1.0 / (1.0 + g);
+    ab64[d] = (1.0 - g) / (1.0 + g);
+    a32[d] = (float)a64[d];
+    ab32[d] = (float)ab64[d];
+  }
+  if ((rc = dalloc(c, &c->d_spa, W)) || (rc = dalloc(c, &c->d_spab, W)) || (rc = dalloc(c, &c->d_spa64, W)) ||
+      (rc = dalloc(c, &c->d_spab64, W)))
+    return bail(rc);
+  if (hipMemcpy(c->d_spa, a32.data(), 4 * W, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_spab, ab32.data(), 4 * W, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_spa64, a64.data(), 8 * W, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_spab64, ab64.data(), 8 * W, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(c, TOMO_ERR_HIP, "upload of sponge tables failed"));
+  // pulse per step, in float and in double arithmetic
+  c->amp32.resize(c->nt);
+  c->amp64.resize(c->nt);
+  for (int k = 0; k < c->nt; ++k) {
+    c->amp32[k] = tomo_host::pulse<float>(*cfg, k);
+    c->amp64[k] = tomo_host::pulse<double>(*cfg, k);
+  }
+  // f = 0 (sirt-mimo.cu:238), receivers of the line layout
+  if (launch_fill_f32(c->d_f, c->n2, 0.f, c->stream) != hipSuccess ||
+      launch_fill_f32(c->d_fo, c->n2, 0.f, c->stream) != hipSuccess)
+    return bail(fail(c, TOMO_ERR_HIP, "initial fill failed"));
+  if ((rc = upload_coefficients(c))) return bail(rc);
+  if (cfg->recv_mode == TOMO_RECV_LINES) {
+    tomo_host::line_receivers(*cfg, c->rcell, c->ranchor);
+  }
+  if ((rc = upload_receivers(c))) return bail(rc);
+  *out = c;
+  return TOMO_OK;
+}
+
+void tomo_destroy(tomo_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto &s : c->spans) {
+    if (s.a) hipEventDestroy(s.a);
+    if (s.b) hipEventDestroy(s.b);
+  }
+  for (auto e : c->ev_free) hipEventDestroy(e);
+  dfree(c->d_f); dfree(c->d_fo); dfree(c->d_pw); dfree(c->d_pw64);
+  dfree(c->d_cellmap); dfree(c->d_anchor); dfree(c->d_bbox);
+  dfree(c->d_spa); dfree(c->d_spab); dfree(c->d_spa64); dfree(c->d_spab64);
+  dfree(c->d_meas); dfree(c->d_rr); dfree(c->d_grad); dfree(c->d_cov); dfree(c->d_misfit);
+  dfree(c->d_scratch); dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_z); dfree(c->d_ring64);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *tomo_last_error(const tomo_ctx *c) {
+  if (!c) return tomo_host::create_error();
+  return c->err.c_str();
+}
+
+int tomo_set_transducers(tomo_ctx *c, const int *ii, const int *jj, int ns, int one_based) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!ii || !jj || ns <= 0) return fail(c, TOMO_ERR_ARG, "bad transducer arrays");
+  const int o = one_based ? 1 : 0;
+  c->tx.resize(ns);
+  c->ty.resize(ns);
+  for (int p = 0; p < ns; ++p) {
+    const int x = ii[p] - o, y = jj[p] - o;
+    if (x < 1 || x > c->nx - 2 || y < 1 || y > c->ny - 2)
+      return fail(c, TOMO_ERR_ARG, "transducer " + std::to_string(p) + " not in the grid interior");
+    c->tx[p] = x;
+    c->ty[p] = y;
+  }
+  c->nshots = 0;
+  return TOMO_OK;
+}
+
+int tomo_set_groups(tomo_ctx *c, int G) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const int ns = (int)c->tx.size();
+  if (ns == 0) return fail(c, TOMO_ERR_STATE, "no transducers");
+  if (G <= 0 || G > ns) return fail(c, TOMO_ERR_ARG, "group size out of range");
+  const int ng = ns / G;
+  c->bbox.resize(ng);
+  for (int g = 0; g < ng; ++g) {
+    const int p = g * G, q = g * G + G - 1;
+    // filled bounding box of the first and last transducer (minmax, sirt-mimo.cu:543-551)
+    c->bbox[g] = make_int4(std::min(c->tx[p], c->tx[q]), std::max(c->tx[p], c->tx[q]), std::min(c->ty[p], c->ty[q]),
+                           std::max(c->ty[p], c->ty[q]));
+  }
+  if ((rc = dalloc(c, &c->d_bbox, ng))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_bbox, c->bbox.data(), sizeof(int4) * ng, hipMemcpyHostToDevice));
+  c->nshots = ng;
+  if ((rc = alloc_shot_buffers(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ng;
+}
+
+int tomo_set_receivers(tomo_ctx *c, const int *ri, const int *rj, int nr, int one_based) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->cfg.recv_mode != TOMO_RECV_POINTS) return fail(c, TOMO_ERR_STATE, "receivers are fixed by the line layout");
+  if (!ri || !rj || nr <= 0) return fail(c, TOMO_ERR_ARG, "bad receiver arrays");
+  const int o = one_based ? 1 : 0;
+  std::vector<int> cells(nr);
+  std::vector<char> seen((size_t)c->n2, 0);
+  for (int r = 0; r < nr; ++r) {
+    const int x = ri[r] - o, y = rj[r] - o;
+    if (x < 2 || x > c->nx - 3 || y < 2 || y > c->ny - 3)
+      return fail(c, TOMO_ERR_ARG, "receiver " + std::to_string(r) + " must lie >= 2 cells inside the edge");
+    const int id = x + c->nx * y;
+    if (seen[id]) return fail(c, TOMO_ERR_ARG, "duplicate receiver cell");
+    seen[id] = 1;
+    cells[r] = id;
+  }
+  c->rcell = cells;
+  c->ranchor = cells;  // additive injection: anchor = the cell itself
+  if ((rc = upload_receivers(c))) return rc;
+  if (c->nshots > 0 && (rc = alloc_shot_buffers(c))) return rc;
+  return TOMO_OK;
+}
+
+int tomo_num_shots(const tomo_ctx *c) { return c ? c->nshots : TOMO_ERR_ARG; }
+int tomo_num_receivers(const tomo_ctx *c) { return c ? c->nr : TOMO_ERR_ARG; }
+
+int tomo_get_receivers(const tomo_ctx *c, int *ri, int *rj) {
+  if (!c || !ri || !rj) return TOMO_ERR_ARG;
+  for (int r = 0; r < c->nr; ++r) {
+    ri[r] = c->rcell[r] % c->nx;
+    rj[r] = c->rcell[r] / c->nx;
+  }
+  return TOMO_OK;
+}
+
+int tomo_set_field(tomo_ctx *c, const float *f) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!f) return fail(c, TOMO_ERR_ARG, "null field");
+  HIPCHK(c, hipMemcpyAsync(c->d_f, f, sizeof(float) * c->n2, hipMemcpyHostToDevice, c->stream));
+  if ((rc = upload_coefficients(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_get_field(tomo_ctx *c, float *f) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!f) return fail(c, TOMO_ERR_ARG, "null field");
+  HIPCHK(c, hipMemcpyAsync(f, c->d_f, sizeof(float) * c->n2, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_set_truth(tomo_ctx *c, const float *fo) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!fo) return fail(c, TOMO_ERR_ARG, "null truth");
+  HIPCHK(c, hipMemcpyAsync(c->d_fo, fo, sizeof(float) * c->n2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_truth = true;
+  return TOMO_OK;
+}
+
+int tomo_forward(tomo_ctx *c, int g0, int ng, int fp64, float *traces) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  if (g0 < 0 || ng <= 0 || g0 + ng > c->nshots) return fail(c, TOMO_ERR_ARG, "shot range out of bounds");
+  const int P = std::min(pass_shots(c), ng);
+  const size_t lvl = (size_t)P * c->n2;
+  if (fp64) {
+    if ((rc = ensure_pw64(c))) return rc;
+    if (c->ring64_n < 3 * lvl) {
+      if ((rc = dalloc(c, &c->d_ring64, 3 * lvl))) return rc;
+      c->ring64_n = 3 * lvl;
+    }
+    rc = forward_traces<double>(c, g0, ng, c->d_ring64, lvl, c->d_pw64, c->d_spa64, c->d_spab64);
+  } else {
+    if ((rc = ensure_wavefields(c, P, false))) return rc;
+    rc = forward_traces<float>(c, g0, ng, c->d_seg, lvl, c->d_pw, c->d_spa, c->d_spab);
+  }
+  if (rc) return rc;
+  // traces now live in d_rr: residuals of those shots must be recomputed
+  for (int g = g0; g < g0 + ng; ++g) c->grad_valid[g] = 0;
+  if (traces) {
+    const size_t tr = (size_t)c->nt * c->nr;
+    HIPCHK(c, hipMemcpyAsync(traces, c->d_rr + (size_t)g0 * tr, sizeof(float) * tr * ng, hipMemcpyDeviceToHost,
+                             c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  spans_collect(c);
+  return TOMO_OK;
+}
+
+int tomo_set_measured(tomo_ctx *c, const float *traces) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  if (!traces) return fail(c, TOMO_ERR_ARG, "null traces");
+  const size_t n = (size_t)c->nshots * c->nt * c->nr;
+  HIPCHK(c, hipMemcpyAsync(c->d_meas, traces, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_meas = true;
+  return TOMO_OK;
+}
+
+int tomo_gradient(tomo_ctx *c, int g0, int ng, double *misfit) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  if (!c->have_meas) return fail(c, TOMO_ERR_STATE, "no measured traces: call tomo_set_measured first");
+  if (g0 < 0 || ng <= 0 || g0 + ng > c->nshots) return fail(c, TOMO_ERR_ARG, "shot range out of bounds");
+  const int P = std::min(pass_shots(c), ng);
+  if ((rc = ensure_wavefields(c, P, true))) return rc;
+  for (int p0 = g0; p0 < g0 + ng; p0 += P) {
+    const int np = std::min(P, g0 + ng - p0);
+    if ((rc = gradient_pass(c, p0, np, P))) return rc;
+  }
+  std::vector<double> mis(ng);
+  HIPCHK(c, hipMemcpyAsync(mis.data(), c->d_misfit + g0, sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  spans_collect(c);
+  for (int g = g0; g < g0 + ng; ++g) c->grad_valid[g] = 1;
+  if (misfit) {
+    double s = 0.0;
+    for (double v : mis) s += v;
+    *misfit = s;
+  }
+  return TOMO_OK;
+}
+
+int tomo_get_gradient(tomo_ctx *c, int g, float *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out || g < 0 || g >= c->nshots) return fail(c, TOMO_ERR_ARG, "bad shot index");
+  if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradient of this shot not computed");
+  HIPCHK(c, hipMemcpyAsync(out, c->d_grad + (size_t)g * c->n2, sizeof(float) * c->n2, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_enable_coverage(tomo_ctx *c, int on) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  c->coverage = on != 0;
+  if (c->coverage && !c->d_cov && c->nshots > 0) {
+    if ((rc = dalloc(c, &c->d_cov, (size_t)c->nshots * c->n2))) return rc;
+    HIPCHK(c, launch_fill_f32(c->d_cov, (size_t)c->nshots * c->n2, 0.f, c->stream));
+  }
+  return TOMO_OK;
+}
+
+int tomo_get_coverage(tomo_ctx *c, float *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) return fail(c, TOMO_ERR_ARG, "null output");
+  if (!c->coverage || !c->d_cov) return fail(c, TOMO_ERR_STATE, "coverage not enabled");
+  std::vector<float> all((size_t)c->nshots * c->n2);
+  HIPCHK(c, hipMemcpyAsync(all.data(), c->d_cov, sizeof(float) * all.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (long long id = 0; id < c->n2; ++id) {
+    double s = 0.0;
+    for (int g = 0; g < c->nshots; ++g) s += all[(size_t)g * c->n2 + id];
+    out[id] = (float)s;
+  }
+  return TOMO_OK;
+}
+
+static int need_all_gradients(tomo_ctx *c) {
+  for (int g = 0; g < c->nshots; ++g)
+    if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradients of all shots are required");
+  return TOMO_OK;
+}
+
+int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
+  const double div = mode == TOMO_UPDATE_MEAN ? (double)c->nshots : 1.0;
+  HIPCHK(c, launch_update_uniform(c->d_f, c->d_grad, c->nshots, c->nx, c->ny, ctx_mask(c), (double)scale, div,
+                                  c->stream));
+  if ((rc = upload_coefficients(c))) return rc;
+  std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
+  const int M = c->nshots;
+  const int pb = weighted_part_blocks((int)c->n2);
+  const size_t need = (size_t)c->n2 + (size_t)M * pb + (size_t)M + 1;
+  if ((rc = ensure_scratch(c, need))) return rc;
+  double *mean = c->d_scratch, *part = mean + c->n2, *w = part + (size_t)M * pb;
+  HIPCHK(c, launch_weighted(c->d_f, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)scale, mean, part, pb, w,
+                            c->stream));
+  if ((rc = upload_coefficients(c))) return rc;
+  std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
+  if (weights_out)
+    HIPCHK(c, hipMemcpyAsync(weights_out, w, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_update_group(tomo_ctx *c, int g, float scale) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  if (g < 0 || g >= c->nshots) return fail(c, TOMO_ERR_ARG, "bad shot index");
+  if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradient of this shot not computed");
+  HIPCHK(c, launch_update_group(c->d_f, c->d_grad + (size_t)g * c->n2, c->nx, c->ny, ctx_mask(c), (double)scale,
+                                c->stream));
+  if ((rc = upload_coefficients(c))) return rc;
+  // the field changed: every stored gradient is stale
+  std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_error(tomo_ctx *c, double *eps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!eps) return fail(c, TOMO_ERR_ARG, "null output");
+  if (!c->have_truth) return fail(c, TOMO_ERR_STATE, "no ground truth: call tomo_set_truth first");
+  if ((rc = ensure_scratch(c, 2))) return rc;
+  HIPCHK(c, launch_error(c->d_f, c->d_fo, (int)c->n2, c->d_scratch, c->stream));
+  double s[2];
+  HIPCHK(c, hipMemcpyAsync(s, c->d_scratch, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *eps = 100.0 * std::sqrt(s[0]) / std::sqrt(s[1]);
+  return TOMO_OK;
+}
+
+int tomo_sirt_iteration(tomo_ctx *c, float scale, int weighted, int mode, double *misfit, double *eps) {
+  int rc = tomo_gradient(c, 0, c ? c->nshots : 0, misfit);
+  if (rc) return rc;
+  rc = weighted ? tomo_update_weighted(c, scale, nullptr) : tomo_update_uniform(c, scale, mode);
+  if (rc) return rc;
+  if (eps && c->have_truth) return tomo_error(c, eps);
+  return TOMO_OK;
+}
+
+int tomo_art_iteration(tomo_ctx *c, float scale, double *misfit, double *eps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  double total = 0.0;
+  for (int g = 0; g < c->nshots; ++g) {
+    double m = 0.0;
+    if ((rc = tomo_gradient(c, g, 1, &m))) return rc;
+    total += m;
+    if ((rc = tomo_update_group(c, g, scale))) return rc;
+  }
+  if (misfit) *misfit = total;
+  if (eps && c->have_truth) return tomo_error(c, eps);
+  return TOMO_OK;
+}
+
+int tomo_traces_to_sides(const tomo_ctx *c, const float *traces, float *bottom, float *right, float *top,
+                         float *left) {
+  if (!c || !traces || !bottom || !right || !top || !left) return TOMO_ERR_ARG;
+  if (c->cfg.recv_mode != TOMO_RECV_LINES) return TOMO_ERR_STATE;
+  return tomo_host::traces_to_sides(c->cfg, c->nshots, traces, bottom, right, top, left);
+}
+
+int tomo_sides_to_traces(const tomo_ctx *c, const float *bottom, const float *right, const float *top,
+                         const float *left, float *traces) {
+  if (!c || !traces || !bottom || !right || !top || !left) return TOMO_ERR_ARG;
+  if (c->cfg.recv_mode != TOMO_RECV_LINES) return TOMO_ERR_STATE;
+  return tomo_host::sides_to_traces(c->cfg, c->nshots, bottom, right, top, left, traces);
+}
+
+int tomo_timing(tomo_ctx *c, double *out6) {
+  if (!c || !out6) return TOMO_ERR_ARG;
+  out6[0] = c->t_ms[0];
+  out6[1] = c->t_launch[0];
+  out6[2] = c->t_ms[1];
+  out6[3] = c->t_launch[1];
+  out6[4] = c->t_cells[0];
+  out6[5] = c->t_cells[1];
+  return TOMO_OK;
+}
+
+int tomo_timing_reset(tomo_ctx *c) {
+  if (!c) return TOMO_ERR_ARG;
+  for (int i = 0; i < 2; ++i) c->t_ms[i] = c->t_launch[i] = c->t_cells[i] = 0;
+  return TOMO_OK;
+}
+
+int tomo_set_tile(tomo_ctx *c, int ry) {
+  if (!c) return TOMO_ERR_ARG;
+  if (ry != 0 && ry != 4 && ry != 8 && ry != 16) return fail(c, TOMO_ERR_ARG, "rows per thread must be 0, 4, 8 or 16");
+  c->tile_ry = ry;
+  return TOMO_OK;
+}
+
+int tomo_get_tile(const tomo_ctx *c) { return c ? tile_rows(c) : TOMO_ERR_ARG; }
+
+}  // extern "C"

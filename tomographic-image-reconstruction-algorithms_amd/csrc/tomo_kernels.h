@@ -1,0 +1,111 @@
+// tomo_kernels.h -- device kernel arguments and launchers of the gfx950
+// wave-tomography engine (internal; the public ABI is include/tomo.h).
+//
+// Data layout in HBM (all i-fastest, rows of nx cells):
+//   wavefield level   float[P][ny][nx]     (P = shots of the current pass)
+//   coefficients pw   T[ny][nx]            shared by every shot
+//   receiver map      int[ny][nx]          receiver index or -1, shared
+//   traces/residuals  float[S][nt][nr]
+//   gradient images   float[S][ny][nx]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tomo {
+
+// Stencil thread block: 64 lanes along x (one wavefront per row of the tile)
+// x 4 wavefronts along y; each lane owns a vertical strip of RY cells.
+constexpr int kBlockX = 64;
+constexpr int kBlockY = 4;
+
+struct Sponge {
+  int width;           // 0: no sponge
+  const void *a;       // T[width]  1/(1+g(d))
+  const void *ab;      // T[width]  (1-g(d))/(1+g(d))
+};
+
+template <typename T>
+struct FwdArgs {
+  const T *um;          // level k-1 of local shot 0
+  const T *u0;          // level k
+  T *up;                // level k+1
+  long long sstride;    // elements between consecutive shots in a level
+  const T *pw;          // a * (v dt / h)^2 per cell
+  const float *f;       // field (edge cells recompute r from it)
+  const int *cellmap;   // receiver index per cell or -1
+  const int4 *bbox;     // per global shot: x0, x1, y0, y1 (inclusive)
+  const float *meas;    // measured traces [S][nt][nr] or null
+  float *rec;           // traces / residuals [S][nt][nr] or null
+  int nx, ny, nt, nr;
+  int tiles_x;
+  int shot0;            // global index of local shot 0
+  int level;            // k + 1
+  int record;           // record this level
+  T amp_h2;             // pulse(k) * h^2, 0 when no source this step
+  T c0dt_h;             // c0 dt / h
+  int sponge_w;
+  const T *sp_a, *sp_ab;
+};
+
+struct AdjArgs {
+  const float *z1;      // adjoint level k+1 of local shot 0
+  const float *z2;      // level k+2
+  float *z0;            // level k (written)
+  const float *u;       // forward level k
+  long long sstride;
+  const float *pw;
+  const int *cellmap;
+  const int *anchor;    // per receiver: anchor cell (flat index)
+  const float *rr;      // residual [S][nt][nr]
+  float *acc;           // gradient accumulators [S][n2] (global shot index)
+  float *cov;           // coverage accumulators [S][n2] or null
+  int nx, ny, nt, nr;
+  int tiles_x;
+  int shot0;
+  int level;            // k
+  float gain;           // residual injection factor
+  float inv_h2;
+  int sponge_w;
+  const float *sp_a, *sp_ab;
+};
+
+// Stencil launchers (grid = tiles x P shots), rows per thread ry in {4, 8, 16}.
+template <typename T>
+hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, int ry, hipStream_t st);
+hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, int ry, hipStream_t st);
+
+// pw = a(d) * (c0 dt / h)^2 * (1 + f)
+template <typename T>
+hipError_t launch_coefficients(const float *f, T *pw, int nx, int ny, double c0dt_h, int sponge_w,
+                               const T *sp_a, hipStream_t st);
+
+hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t st);
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st);
+
+// per-shot sum of squares of rr[s][nt][nr] in FP64 -> out[s]
+hipError_t launch_misfit(const float *rr, long long per_shot, int nshots, double *out, hipStream_t st);
+
+// Mask rectangle for the update kernels.
+struct Mask {
+  int x0, x1, y0, y1;
+};
+
+// f += scale * mask * (sum_g G_g) / (1 + f) / div
+hipError_t launch_update_uniform(float *f, const float *G, int nshots, int nx, int ny, Mask m,
+                                 double scale, double div, hipStream_t st);
+// f += scale * mask * G_g / (1 + f)
+hipError_t launch_update_group(float *f, const float *Gg, int nx, int ny, Mask m, double scale,
+                               hipStream_t st);
+// weighted SIRT: mean image (FP64), per-shot squared distance partials,
+// weights (normalised, degenerate fallback), update.
+hipError_t launch_weighted(float *f, const float *G, int M, int nx, int ny, Mask m, double scale,
+                           double *mean_scratch, double *part_scratch, int part_blocks,
+                           double *w_out, hipStream_t st);
+// sums {sum (f - fo)^2, sum fo^2} in FP64 into out[2]
+hipError_t launch_error(const float *f, const float *fo, int n, double *out, hipStream_t st);
+
+// number of partial blocks used by launch_weighted for n2 cells
+int weighted_part_blocks(int n2);
+
+}  // namespace tomo

@@ -1,0 +1,520 @@
+// tomo_kernels.hip -- gfx950 kernels of the wave-tomography engine.
+//
+// Stencil kernels: a 256-thread block covers a 64 x (4*RY) tile of one shot;
+// lane x of a wavefront reads row-contiguous, 256-byte coalesced cells, and
+// each lane walks a vertical strip of RY cells keeping the rows above /
+// below in registers, so a cell's y-neighbours are never re-read.  Blocks
+// of all shots of a pass are launched together (grid.y = shot).  Cells on
+// the outer edge, and receiver cells in the adjoint, take a slow path with
+// direct loads; they are O(n) of the O(n^2) cells.
+//
+// Reductions are FP64 with a fixed summation order (deterministic).
+#include "tomo_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace tomo {
+
+namespace {
+
+__device__ __forceinline__ int edge_dist(int x, int y, int nx, int ny) {
+  return min(min(x, y), min(nx - 1 - x, ny - 1 - y));
+}
+
+// ----------------------------------------------------------------- forward
+
+// One-way edge rule of propagation (sirt-mimo.cu:566-587): corners use the
+// diagonal neighbour.
+template <typename T>
+__device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T *__restrict__ um,
+                          int x, int y) {
+  const int nx = A.nx, ny = A.ny;
+  const int id = y * nx + x;
+  const T r = A.c0dt_h * sqrt(T(1) + T(A.f[id]));
+  const bool top = y == 0, bottom = y == ny - 1, left = x == 0, right = x == nx - 1;
+  const int ja = top ? y + 1 : bottom ? y - 1 : y;
+  const int jb = top ? y + 2 : bottom ? y - 2 : y;
+  const int ia = left ? x + 1 : right ? x - 1 : x;
+  const int ib = left ? x + 2 : right ? x - 2 : x;
+  const int ida = ja * nx + ia, idb = jb * nx + ib;
+  return (T(2) - T(2) * r - r * r) * u0[id] + T(2) * r * (T(1) + r) * u0[ida] - r * r * u0[idb] +
+         (T(2) * r - T(1)) * um[id] - T(2) * r * um[ida];
+}
+
+// Leapfrog step k -> k+1 for every cell of one tile of one shot
+// (propagation, sirt-mimo.cu:511-593, and the receiver record of
+// initial_signal / difference_signal, sirt-mimo.cu:617-684, fused).
+template <typename T, bool SPONGE, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<T> A) {
+  const int nx = A.nx, ny = A.ny;
+  const int tile = blockIdx.x;
+  const int x = (tile % A.tiles_x) * kBlockX + threadIdx.x;
+  const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
+  if (x >= nx || y0 >= ny) return;
+  const int gs = A.shot0 + blockIdx.y;
+  const long long so = (long long)blockIdx.y * A.sstride;
+  const T *__restrict__ u0 = A.u0 + so;
+  const T *__restrict__ um = A.um + so;
+  T *__restrict__ up = A.up + so;
+  const T *__restrict__ pw = A.pw;
+  const int4 bb = A.bbox[gs];
+  const bool src_col = (A.amp_h2 != T(0)) && x >= bb.x && x <= bb.y;
+  const bool xin = x > 0 && x < nx - 1;
+
+  T cu = y0 > 0 ? u0[(y0 - 1) * nx + x] : T(0);
+  T cc = u0[y0 * nx + x];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + r;
+    if (y >= ny) break;
+    const int id = y * nx + x;
+    const T cd = (y + 1 < ny) ? u0[id + nx] : T(0);
+    T val;
+    if (xin && y > 0 && y < ny - 1) {
+      const T l = u0[id - 1], rt = u0[id + 1];
+      const T p = pw[id];
+      const T sum = ((rt + l) + cu) + cd;  // u(i+1) + u(i-1) + u(j-1) + u(j+1)
+      const T prev = um[id];
+      if (SPONGE) {
+        const int d = edge_dist(x, y, nx, ny);
+        T a = T(1), ab = T(1);
+        if (d < A.sponge_w) {
+          a = A.sp_a[d];
+          ab = A.sp_ab[d];
+        }
+        val = p * sum + (T(2) * a - T(4) * p) * cc - ab * prev;
+      } else {
+        val = p * sum + (T(2) - T(4) * p) * cc - prev;
+      }
+      if (src_col && y >= bb.z && y <= bb.w) val += p * A.amp_h2;
+    } else {
+      val = forward_edge(A, u0, um, x, y);
+    }
+    up[id] = val;
+    if (A.record) {
+      const int ri = A.cellmap[id];
+      if (ri >= 0) {
+        const long long o = ((long long)gs * A.nt + A.level) * A.nr + ri;
+        A.rec[o] = A.meas ? A.meas[o] - float(val) : float(val);
+      }
+    }
+    cu = cc;
+    cc = cd;
+  }
+}
+
+// ----------------------------------------------------------------- adjoint
+
+template <bool SPONGE>
+__device__ __forceinline__ void sponge_factors(const AdjArgs &A, int x, int y, float &a, float &ab) {
+  a = 1.f;
+  ab = 1.f;
+  if (SPONGE) {
+    const int d = edge_dist(x, y, A.nx, A.ny);
+    if (d < A.sponge_w) {
+      a = A.sp_a[d];
+      ab = A.sp_ab[d];
+    }
+  }
+}
+
+// Interior adjoint stencil at (x, y) (backpropagation1, sirt-mimo.cu:686-710,
+// written with the per-cell weight pw = c0^2 dt^2 (1 + f) / h^2).
+template <bool SPONGE>
+__device__ float adjoint_interior(const AdjArgs &A, const float *__restrict__ z1,
+                                  const float *__restrict__ z2, int x, int y) {
+  const int nx = A.nx;
+  const int id = y * nx + x;
+  const float *__restrict__ pw = A.pw;
+  float a, ab;
+  sponge_factors<SPONGE>(A, x, y, a, ab);
+  return (((pw[id - nx] * z1[id - nx] + pw[id + nx] * z1[id + nx]) + pw[id - 1] * z1[id - 1]) +
+          pw[id + 1] * z1[id + 1]) +
+         (2.f * a - 4.f * pw[id]) * z1[id] - ab * z2[id];
+}
+
+// forward level with the corner cells replaced by the mean of their two edge
+// neighbours (propagation_at_corners, sirt-mimo.cu:595-615)
+__device__ __forceinline__ float u_corner_mean(const float *__restrict__ u, int nx, int ny, int x, int y) {
+  const bool cx = (x == 0 || x == nx - 1), cy = (y == 0 || y == ny - 1);
+  if (cx && cy) {
+    const int xi = x == 0 ? 1 : nx - 2, yi = y == 0 ? 1 : ny - 2;
+    return 0.5f * (u[yi * nx + x] + u[y * nx + xi]);
+  }
+  return u[y * nx + x];
+}
+
+// clamped 5-point Laplacian (laplace, sirt-mimo.cu:776-804) for a non-corner cell
+__device__ float lap_clamped(const float *__restrict__ u, int nx, int ny, int x, int y, float inv_h2) {
+  const int ya = y > 0 ? y - 1 : y, yb = y < ny - 1 ? y + 1 : y;
+  const int xa = x > 0 ? x - 1 : x, xb = x < nx - 1 ? x + 1 : x;
+  return (((u_corner_mean(u, nx, ny, x, ya) + u_corner_mean(u, nx, ny, x, yb)) +
+           u_corner_mean(u, nx, ny, xa, y)) +
+          u_corner_mean(u, nx, ny, xb, y) - 4.f * u_corner_mean(u, nx, ny, x, y)) *
+         inv_h2;
+}
+
+// Slow path: edge, corner and receiver cells.  Final z(k) follows
+// backpropagation2 (sirt-mimo.cu:712-774): receiver = inward anchor + gain *
+// residual, edge = copy of the inward cell, corner = mean of the two edge
+// copies (= the diagonal interior cell); Lu corners are the mean of the edge
+// neighbours' Lu (laplace_corners, sirt-mimo.cu:806-831).
+template <bool SPONGE>
+__device__ void adjoint_slow(const AdjArgs &A, const float *__restrict__ z1, const float *__restrict__ z2,
+                             const float *__restrict__ u, int gs, int x, int y, int ri, float &zval,
+                             float &lu, float &uval) {
+  const int nx = A.nx, ny = A.ny;
+  const bool ex = (x == 0 || x == nx - 1), ey = (y == 0 || y == ny - 1);
+  const int xi = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
+  const int yi = y == 0 ? 1 : (y == ny - 1 ? ny - 2 : y);
+  if (ex || ey) {
+    zval = adjoint_interior<SPONGE>(A, z1, z2, xi, yi);
+  } else {
+    const int an = A.anchor[ri];
+    zval = adjoint_interior<SPONGE>(A, z1, z2, an % nx, an / nx) +
+           A.rr[((long long)gs * A.nt + A.level) * A.nr + ri] * A.gain;
+  }
+  if (ex && ey) {
+    lu = (lap_clamped(u, nx, ny, xi, y, A.inv_h2) + lap_clamped(u, nx, ny, x, yi, A.inv_h2)) / 2.f;
+  } else {
+    lu = lap_clamped(u, nx, ny, x, y, A.inv_h2);
+  }
+  uval = u_corner_mean(u, nx, ny, x, y);
+}
+
+// Adjoint step k+1, k+2 -> k with the correlation acc += z(k) * Lu(k)
+// fused (backpropagation1/2 + laplace + update_differential,
+// sirt-mimo-io.cu:342-353): the adjoint field is consumed while it is in
+// registers, so z and Lu histories are never stored.
+template <bool SPONGE, bool COV, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_step_kernel(AdjArgs A) {
+  const int nx = A.nx, ny = A.ny;
+  const int tile = blockIdx.x;
+  const int x = (tile % A.tiles_x) * kBlockX + threadIdx.x;
+  const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
+  if (x >= nx || y0 >= ny) return;
+  const int gs = A.shot0 + blockIdx.y;
+  const long long so = (long long)blockIdx.y * A.sstride;
+  const long long n2 = (long long)nx * ny;
+  const float *__restrict__ z1 = A.z1 + so;
+  const float *__restrict__ z2 = A.z2 + so;
+  const float *__restrict__ u = A.u + so;
+  float *__restrict__ z0 = A.z0 + so;
+  float *__restrict__ acc = A.acc + gs * n2;
+  float *__restrict__ cov = COV ? A.cov + gs * n2 : nullptr;
+  const float *__restrict__ pw = A.pw;
+  const bool xin = x > 0 && x < nx - 1;
+
+  const int idt = y0 * nx + x;
+  float zu = y0 > 0 ? z1[idt - nx] : 0.f, zc = z1[idt];
+  float pu = y0 > 0 ? pw[idt - nx] : 0.f, pc = pw[idt];
+  float uu = y0 > 0 ? u[idt - nx] : 0.f, uc = u[idt];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = y0 + r;
+    if (y >= ny) break;
+    const int id = y * nx + x;
+    const bool down = y + 1 < ny;
+    const float zd = down ? z1[id + nx] : 0.f;
+    const float pd = down ? pw[id + nx] : 0.f;
+    const float ud = down ? u[id + nx] : 0.f;
+    const bool interior = xin && y > 0 && y < ny - 1;
+    const int ri = interior ? A.cellmap[id] : -1;
+    float zval, lu, uval;
+    if (interior && ri < 0) {
+      float a, ab;
+      sponge_factors<SPONGE>(A, x, y, a, ab);
+      const float zl = z1[id - 1], zr = z1[id + 1];
+      const float pl = pw[id - 1], pr = pw[id + 1];
+      zval = (((pu * zu + pd * zd) + pl * zl) + pr * zr) + (2.f * a - 4.f * pc) * zc - ab * z2[id];
+      const float ul = u[id - 1], ur = u[id + 1];
+      lu = (((uu + ud) + ul) + ur - 4.f * uc) * A.inv_h2;
+      uval = uc;
+    } else {
+      adjoint_slow<SPONGE>(A, z1, z2, u, gs, x, y, ri, zval, lu, uval);
+    }
+    z0[id] = zval;
+    acc[id] += zval * lu;
+    if (COV) cov[id] += uval * uval;
+    zu = zc; zc = zd;
+    pu = pc; pc = pd;
+    uu = uc; uc = ud;
+  }
+}
+
+// ------------------------------------------------------------- elementwise
+
+template <typename T>
+__global__ void coefficients_kernel(const float *__restrict__ f, T *__restrict__ pw, int nx, int ny,
+                                    T c0dt_h, int sponge_w, const T *__restrict__ sp_a) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny) return;
+  const T r = c0dt_h * sqrt(T(1) + T(f[id]));
+  T a = T(1);
+  if (sponge_w > 0) {
+    const int d = edge_dist(id % nx, id / nx, nx, ny);
+    if (d < sponge_w) a = sp_a[d];
+  }
+  pw[id] = a * (r * r);
+}
+
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ p, size_t n, T v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Fixed-order block sum of one double per thread (blockDim.x = 256).
+__device__ double block_sum_256(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void misfit_kernel(const float *__restrict__ rr, long long per_shot,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[256];
+  const float *p = rr + blockIdx.x * per_shot;
+  double acc = 0.0;
+  for (long long i = threadIdx.x; i < per_shot; i += 256) {
+    const double v = p[i];
+    acc += v * v;
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+__device__ __forceinline__ bool in_mask(const Mask &m, int x, int y) {
+  return x >= m.x0 && x < m.x1 && y >= m.y0 && y < m.y1;
+}
+
+// uniform SIRT (sirt-mimo.cu:858-877; sirt-mimo-io.cu:774-796)
+__global__ void update_uniform_kernel(float *__restrict__ f, const float *__restrict__ G, int S, int nx,
+                                      int ny, Mask m, double scale, double div) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n2 = (long long)nx * ny;
+  if (id >= n2 || !in_mask(m, id % nx, id / nx)) return;
+  double s = 0.0;
+  for (int g = 0; g < S; ++g) s += (double)G[g * n2 + id];
+  const double fv = f[id];
+  f[id] = (float)(fv + scale * (s / (1.0 + fv)) / div);
+}
+
+// ART step (art-mimo-io.cu:381-400)
+__global__ void update_group_kernel(float *__restrict__ f, const float *__restrict__ Gg, int nx, int ny,
+                                    Mask m, double scale) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * ny || !in_mask(m, id % nx, id / nx)) return;
+  const double fv = f[id];
+  f[id] = (float)(fv + scale * ((double)Gg[id] / (1.0 + fv)));
+}
+
+// weighted SIRT, pass 1: mean increment image (weights_differential1's
+// df_avg / Ng, sirt-mimo-norm.cu:56-74)
+__global__ void weighted_mean_kernel(const float *__restrict__ f, const float *__restrict__ G, int M,
+                                     long long n2, double *__restrict__ mean) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n2) return;
+  const double q = 1.0 + (double)f[id];
+  double s = 0.0;
+  for (int m = 0; m < M; ++m) s += (double)G[m * n2 + id] / q;
+  mean[id] = s / M;
+}
+
+// pass 2: partial squared distances of each shot's increment to the mean
+__global__ __launch_bounds__(256) void weighted_norm_kernel(const float *__restrict__ f,
+                                                            const float *__restrict__ G,
+                                                            const double *__restrict__ mean, long long n2,
+                                                            int chunk, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int m = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, n2);
+  double acc = 0.0;
+  for (long long id = beg + threadIdx.x; id < end; id += 256) {
+    const double v = (double)G[m * n2 + id] / (1.0 + (double)f[id]) - mean[id];
+    acc += v * v;
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) part[(long long)m * gridDim.x + blockIdx.x] = s;
+}
+
+// pass 3: d_m = sqrt(sum parts); w_m = d_m^-1 / sum d^-1, uniform 1/M when a
+// d_m is zero (weights_differential2, sirt-mimo-norm.cu:76-97)
+__global__ __launch_bounds__(256) void weighted_weights_kernel(const double *__restrict__ part, int M,
+                                                               int nparts, double *__restrict__ w) {
+  __shared__ int degenerate;
+  if (threadIdx.x == 0) degenerate = 0;
+  __syncthreads();
+  for (int m = threadIdx.x; m < M; m += 256) {
+    double d2 = 0.0;
+    for (int b = 0; b < nparts; ++b) d2 += part[(long long)m * nparts + b];
+    if (!(d2 > 0.0)) atomicOr(&degenerate, 1);
+    w[m] = 1.0 / sqrt(d2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double W = 0.0;
+    for (int m = 0; m < M; ++m) W += w[m];
+    if (degenerate || !isfinite(W)) degenerate = 1;
+    else degenerate = 0;
+    w[M] = W;  // scratch slot
+  }
+  __syncthreads();
+  const double W = w[M];
+  const int deg = degenerate;
+  __syncthreads();
+  for (int m = threadIdx.x; m < M; m += 256) w[m] = deg ? 1.0 / M : w[m] / W;
+}
+
+// pass 4: f += scale * mask * sum_m w_m df_m (average_differential + update_field,
+// sirt-mimo-norm.cu:100-118, 873-892)
+__global__ void weighted_update_kernel(float *__restrict__ f, const float *__restrict__ G, int M, int nx,
+                                       int ny, Mask msk, double scale, const double *__restrict__ w) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n2 = (long long)nx * ny;
+  if (id >= n2 || !in_mask(msk, id % nx, id / nx)) return;
+  const double q = 1.0 + (double)f[id];
+  double s = 0.0;
+  for (int m = 0; m < M; ++m) s += w[m] * ((double)G[m * n2 + id] / q);
+  f[id] = (float)((double)f[id] + scale * s);
+}
+
+__global__ __launch_bounds__(256) void error_kernel(const float *__restrict__ f, const float *__restrict__ fo,
+                                                    int n, double *__restrict__ out) {
+  __shared__ double sh[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double d = (double)f[i] - (double)fo[i];
+    a += d * d;
+    b += (double)fo[i] * (double)fo[i];
+  }
+  const double sa = block_sum_256(a, sh);
+  const double sb = block_sum_256(b, sh);
+  if (threadIdx.x == 0) {
+    out[0] = sa;
+    out[1] = sb;
+  }
+}
+
+inline dim3 stencil_grid(int nx, int ny, int ry, int nshots, int *tiles_x) {
+  *tiles_x = (nx + kBlockX - 1) / kBlockX;
+  const int tiles_y = (ny + kBlockY * ry - 1) / (kBlockY * ry);
+  return dim3(*tiles_x * tiles_y, nshots, 1);
+}
+
+template <typename T, bool SPONGE>
+hipError_t fwd_dispatch(FwdArgs<T> a, int nshots, int ry, hipStream_t st) {
+  const dim3 block(kBlockX, kBlockY, 1);
+  const dim3 grid = stencil_grid(a.nx, a.ny, ry, nshots, &a.tiles_x);
+  switch (ry) {
+    case 4: forward_step_kernel<T, SPONGE, 4><<<grid, block, 0, st>>>(a); break;
+    case 16: forward_step_kernel<T, SPONGE, 16><<<grid, block, 0, st>>>(a); break;
+    default: forward_step_kernel<T, SPONGE, 8><<<grid, block, 0, st>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+template <bool SPONGE, bool COV>
+hipError_t adj_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
+  const dim3 block(kBlockX, kBlockY, 1);
+  const dim3 grid = stencil_grid(a.nx, a.ny, ry, nshots, &a.tiles_x);
+  switch (ry) {
+    case 4: adjoint_step_kernel<SPONGE, COV, 4><<<grid, block, 0, st>>>(a); break;
+    case 16: adjoint_step_kernel<SPONGE, COV, 16><<<grid, block, 0, st>>>(a); break;
+    default: adjoint_step_kernel<SPONGE, COV, 8><<<grid, block, 0, st>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+template <typename T>
+hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, int ry, hipStream_t st) {
+  if (ry != 4 && ry != 8 && ry != 16) ry = 8;
+  return a.sponge_w > 0 ? fwd_dispatch<T, true>(a, nshots, ry, st) : fwd_dispatch<T, false>(a, nshots, ry, st);
+}
+template hipError_t launch_forward_step<float>(const FwdArgs<float> &, int, int, hipStream_t);
+template hipError_t launch_forward_step<double>(const FwdArgs<double> &, int, int, hipStream_t);
+
+hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, int ry, hipStream_t st) {
+  if (ry != 4 && ry != 8 && ry != 16) ry = 8;
+  const bool sp = a.sponge_w > 0, cov = a.cov != nullptr;
+  if (sp) return cov ? adj_dispatch<true, true>(a, nshots, ry, st) : adj_dispatch<true, false>(a, nshots, ry, st);
+  return cov ? adj_dispatch<false, true>(a, nshots, ry, st) : adj_dispatch<false, false>(a, nshots, ry, st);
+}
+
+template <typename T>
+hipError_t launch_coefficients(const float *f, T *pw, int nx, int ny, double c0dt_h, int sponge_w,
+                               const T *sp_a, hipStream_t st) {
+  const int n = nx * ny;
+  coefficients_kernel<T><<<(n + 255) / 256, 256, 0, st>>>(f, pw, nx, ny, T(c0dt_h), sponge_w, sp_a);
+  return hipGetLastError();
+}
+template hipError_t launch_coefficients<float>(const float *, float *, int, int, double, int, const float *,
+                                               hipStream_t);
+template hipError_t launch_coefficients<double>(const float *, double *, int, int, double, int, const double *,
+                                                hipStream_t);
+
+hipError_t launch_fill_f32(float *p, size_t n, float v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  fill_kernel<float><<<blocks, 256, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  fill_kernel<double><<<blocks, 256, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_misfit(const float *rr, long long per_shot, int nshots, double *out, hipStream_t st) {
+  if (nshots <= 0) return hipSuccess;
+  misfit_kernel<<<nshots, 256, 0, st>>>(rr, per_shot, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_uniform(float *f, const float *G, int nshots, int nx, int ny, Mask m, double scale,
+                                 double div, hipStream_t st) {
+  const int n = nx * ny;
+  update_uniform_kernel<<<(n + 255) / 256, 256, 0, st>>>(f, G, nshots, nx, ny, m, scale, div);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_group(float *f, const float *Gg, int nx, int ny, Mask m, double scale,
+                               hipStream_t st) {
+  const int n = nx * ny;
+  update_group_kernel<<<(n + 255) / 256, 256, 0, st>>>(f, Gg, nx, ny, m, scale);
+  return hipGetLastError();
+}
+
+int weighted_part_blocks(int n2) { return std::max(1, std::min(256, (n2 + 4095) / 4096)); }
+
+hipError_t launch_weighted(float *f, const float *G, int M, int nx, int ny, Mask m, double scale,
+                           double *mean, double *part, int part_blocks, double *w, hipStream_t st) {
+  const long long n2 = (long long)nx * ny;
+  const int nb = (int)((n2 + 255) / 256);
+  weighted_mean_kernel<<<nb, 256, 0, st>>>(f, G, M, n2, mean);
+  const int chunk = (int)((n2 + part_blocks - 1) / part_blocks);
+  weighted_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, n2, chunk, part);
+  weighted_weights_kernel<<<1, 256, 0, st>>>(part, M, part_blocks, w);
+  weighted_update_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_error(const float *f, const float *fo, int n, double *out, hipStream_t st) {
+  error_kernel<<<1, 256, 0, st>>>(f, fo, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace tomo
