@@ -5,7 +5,8 @@ Tolerances (relative L2 against the FP64 oracle) come from the FP32
 arithmetic itself: the oracle run in FP32 differs from its FP64 run by up to
 1e-4 on traces, 3.3e-3 on a single shot's gradient (the time sum
 of z * Lu cancels heavily) and 3e-4 on the summed gradient; the engine is
-allowed twice that.  The FP64 solver must agree to 1e-11.  Checkpointed,
+allowed twice that.  The FP64 solver must agree to the float rounding of
+its returned traces (1e-7).  Checkpointed,
 full-history and shot-pass runs of the engine must be bitwise identical."""
 import numpy as np
 import pytest
@@ -18,7 +19,8 @@ from _pkg import tomo
 pytestmark = pytest.mark.gpu
 
 TR32 = 2e-4      # traces, FP32 engine vs FP64 oracle
-TR64 = 1e-11     # traces, FP64 engine vs FP64 oracle
+TR64 = 1e-7      # traces, FP64 engine vs FP64 oracle: the traces come back as float, so
+                  # float rounding (6e-8) bounds it; the FP32 solver lands at ~3e-6
 G_SHOT = 8e-3    # per-shot gradient
 G_SUM = 1e-3     # summed gradient
 MISFIT = 1e-4

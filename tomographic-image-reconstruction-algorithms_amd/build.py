@@ -72,7 +72,7 @@ def build_drivers(verbose: bool = False) -> list:
     extra = [common] if os.path.exists(common) else []
 
     def one(src):
-        exe = os.path.join(OUT, src[:-4])
+        exe = os.path.join(OUT, src[:-4].replace("_", "-"))
         _run([CXX, "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", DRIVERS, os.path.join(DRIVERS, src), *extra,
               "-o", exe, "-L", OUT, "-ltomo_hip", f"-Wl,-rpath,$ORIGIN"], verbose)
         return exe
