@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
-    ap.add_argument("--tile", type=int, default=0, help="rows per thread (0 = auto)")
+    ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     args = ap.parse_args()
@@ -96,8 +96,8 @@ def main():
     c, ii, jj, fo = make_workload(args.workload)
     n, steps, shots, ckpt = WORKLOADS[args.workload]
     e = tomo.Engine(c, device=0)
-    if args.tile:
-        e.set_tile(args.tile)
+    if args.shape:
+        e.set_shape(*[int(v) for v in args.shape.split(",")])
     e.set_transducers(ii, jj, one_based=False)
     e.set_receivers(ii, jj, one_based=False)
     e.set_groups(1)
@@ -153,7 +153,7 @@ def main():
             "workload": f"batched plain SIRT iteration: {n}x{n} grid, {shots} ring shots, {steps} steps, "
                         f"checkpoint every {ckpt or 'none'}",
             "grid": n, "shots": shots, "time_steps": steps, "checkpoint_every": ckpt,
-            "cell_updates_per_step": cells / args.steps, "tile_rows": e.get_tile(),
+            "cell_updates_per_step": cells / args.steps, "stencil_shape": list(e.get_shape()),
             "eps_after_steps": eps[-1] if eps else None,
         },
         "roofline": {

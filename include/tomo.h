@@ -293,9 +293,14 @@ int tomo_read_text(const char *path, float *a, int nx, int ny, int nz, int nw);
 int tomo_timing(tomo_ctx *ctx, double *out6);
 int tomo_timing_reset(tomo_ctx *ctx);
 
-/* Tile shape of the stencil kernels (rows per thread); 0 = autotuned. */
-int tomo_set_tile(tomo_ctx *ctx, int rows_per_thread);
-int tomo_get_tile(const tomo_ctx *ctx);
+/* Shape of the stencil kernels: vec = cells per lane along x (1 scalar, 2 or
+ * 4 vectorised; FP32 solver only, nx % vec == 0), rows per lane along y for
+ * the forward step (vec 1: 4/8/16; vec 2/4: 2/4/8) and the adjoint step
+ * (vec 1: 4/8; vec 2/4: 1/2/4).  0 in a field picks the default.  Every shape
+ * gives the same results up to FP32 rounding.  No reference counterpart (its
+ * launch shapes are fixed, sirt-mimo.cu:165-299). */
+int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows);
+int tomo_get_shape(const tomo_ctx *ctx, int *vec, int *fwd_rows, int *adj_rows);
 
 #ifdef __cplusplus
 }

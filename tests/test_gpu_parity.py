@@ -170,18 +170,58 @@ def test_checkpoint_and_shot_passes_are_bitwise_identical():
         np.testing.assert_array_equal(G, results[0][1])
 
 
-def test_tile_shapes_agree():
-    say("tiles ref 64")
-    c = ref_config(64, 161)
+SHAPES = [(1, 4, 4), (1, 8, 8), (1, 16, 4), (2, 2, 1), (2, 4, 2), (2, 8, 4), (4, 2, 1), (4, 4, 2), (4, 8, 4)]
+
+
+@pytest.mark.parametrize("n", [64, 192])
+def test_stencil_shapes_agree(n):
+    """Every kernel shape (scalar or vectorised, rows per lane) gives the
+    oracle's traces and gradients; shapes of one vector width agree bitwise."""
+    say("shapes ref", n)
+    nt = 161 if n == 64 else 401   # the scattered wave needs ~320 levels to come back at 192
+    c = ref_config(n, nt)
     fo = ref_phantom(c)
-    outs = []
-    for ry in (4, 8, 16):
-        e, c2, bbox = ref_engine(64, 161, 8, field=fo)
-        e.set_tile(ry)
-        outs.append(e.forward())
+    G = 8 if n == 64 else 32
+    rc, ra = orc.receivers_lines(c)
+    _, _, bbox = ref_geometry(c, G)
+    meas = orc.forward(c, fo, bbox, rc, np.float64)
+    Gw, _ = orc.gradient(c, np.zeros(fo.size), bbox, rc, ra, meas, np.float64)
+    by_vec = {}
+    for vec, fr, ar in SHAPES:
+        e, c2, _ = ref_engine(n, nt, G, field=fo)
+        e.set_shape(vec, fr, ar)
+        assert e.get_shape() == (vec, fr, ar)
+        tr = e.forward()
+        e.set_field(np.zeros(fo.size, np.float32))
+        e.set_measured(meas.astype(np.float32))
+        e.gradient()
+        Gg = np.stack([e.get_gradient(g) for g in range(e.num_shots)])
         e.close()
-    np.testing.assert_array_equal(outs[0], outs[1])
-    np.testing.assert_array_equal(outs[0], outs[2])
+        assert rel_l2(tr, meas) < TR32, (vec, fr, ar)
+        assert rel_l2(Gg.sum(0), Gw.sum(0)) < G_SUM, (vec, fr, ar)
+        if vec in by_vec:
+            np.testing.assert_array_equal(tr, by_vec[vec][0])
+            np.testing.assert_array_equal(Gg, by_vec[vec][1])
+        else:
+            by_vec[vec] = (tr, Gg)
+
+
+def test_vectorised_unit_preset():
+    say("shapes unit 96")
+    e, c, bbox, ii, jj = unit_engine(96, 300)
+    fo = unit_phantom(c)
+    rc, ra = unit_receivers(c, ii, jj)
+    meas = orc.forward(c, fo, bbox, rc, np.float64)
+    Gw, _ = orc.gradient(c, np.zeros(fo.size), bbox, rc, ra, meas, np.float64)
+    for vec, fr, ar in [(2, 4, 2), (4, 4, 1)]:
+        e.set_shape(vec, fr, ar)
+        e.set_field(fo)
+        assert rel_l2(e.forward(), meas) < TR32
+        e.set_field(np.zeros(fo.size, np.float32))
+        e.set_measured(meas.astype(np.float32))
+        e.gradient()
+        Gg = np.stack([e.get_gradient(g) for g in range(e.num_shots)])
+        assert rel_l2(Gg.sum(0), Gw.sum(0)) < G_SUM
 
 
 # ------------------------------------------------------------------ updates

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "tomo.h"
@@ -65,7 +66,7 @@ struct tomo_ctx {
   std::vector<char> grad_valid;
   std::vector<float> amp32;   // pulse per step, float arithmetic
   std::vector<double> amp64;  // pulse per step, double arithmetic
-  int tile_ry = 0;
+  StencilShape shape_req{0, 0, 0, 0};  // requested (0 = automatic)
   // timing
   std::vector<hipEvent_t> ev_free;
   std::vector<TimedSpan> spans;
@@ -117,11 +118,25 @@ int pass_shots(const tomo_ctx *c) {
   return (P > 0 && P < c->nshots) ? P : std::max(c->nshots, 1);
 }
 
-// rows per thread of the stencil tiles
-int tile_rows(const tomo_ctx *c) {
-  if (c->tile_ry) return c->tile_ry;
-  // taller strips amortise the strip halo; short grids keep enough blocks
-  return c->ny >= 256 ? 16 : 8;
+// Stencil kernel shape: the vectorised kernels (4 or 2 cells per lane) when
+// the row length allows, with the requested fields overriding the defaults.
+StencilShape stencil_shape(const tomo_ctx *c) {
+  StencilShape s;
+  s.vec = c->shape_req.vec;
+  // 4-wide lanes when a 256-column tile is filled (or nearly), else 2-wide
+  if (s.vec == 0) s.vec = (c->nx % 4 == 0 && (c->nx % 256 == 0 || c->nx > 256)) ? 4 : (c->nx % 2 == 0 ? 2 : 1);
+  if (c->nx % s.vec != 0) s.vec = 1;
+  // measured best on 512 x 512 x 128 shots: 2 rows per lane forward, 1 adjoint
+  s.fwd_rows = c->shape_req.fwd_rows ? c->shape_req.fwd_rows : (s.vec > 1 ? 2 : 4);
+  s.adj_rows = c->shape_req.adj_rows ? c->shape_req.adj_rows : (s.vec > 1 ? 1 : 4);
+  s.rows = s.fwd_rows;
+  return s;
+}
+
+bool shape_valid(int vec, int fr, int ar) {
+  if (vec != 0 && vec != 1 && vec != 2 && vec != 4) return false;
+  if (vec == 1) return (fr == 0 || fr == 4 || fr == 8 || fr == 16) && (ar == 0 || ar == 4 || ar == 8);
+  return (fr == 0 || fr == 2 || fr == 4 || fr == 8) && (ar == 0 || ar == 1 || ar == 2 || ar == 4);
 }
 
 hipEvent_t take_event(tomo_ctx *c) {
@@ -347,7 +362,11 @@ int forward_step(tomo_ctx *c, FwdArgs<T> a, const T *um, const T *u0, T *up, int
   a.meas = meas;
   a.rec = rec;
   a.amp_h2 = pulse_h2<T>(c, k);
-  HIPCHK(c, launch_forward_step<T>(a, P, tile_rows(c), c->stream));
+  StencilShape sh = stencil_shape(c);
+  if (!std::is_same<T, float>::value) sh.vec = 1;  // the FP64 solver runs the scalar kernels
+  if (sh.vec == 1 && sh.fwd_rows < 4) sh.fwd_rows = 4;
+  sh.rows = sh.fwd_rows;
+  HIPCHK(c, launch_forward_step<T>(a, P, sh, c->stream));
   return TOMO_OK;
 }
 
@@ -437,6 +456,8 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   const int C = full ? nt : c->cfg.checkpoint_every;
   const int nseg = (nt + C - 1) / C;
   AdjArgs aa = adj_args(c, p0);
+  StencilShape ash = stencil_shape(c);
+  ash.rows = ash.adj_rows;
   for (int m = nseg - 1; m >= 0; --m) {
     const int k0 = m * C;
     const int cnt = std::min(C, nt - k0);
@@ -464,7 +485,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
       aa.z0 = Z(k);
       aa.u = segment_level(c, P, m, k);
       aa.level = k;
-      HIPCHK(c, launch_adjoint_step(aa, np, tile_rows(c), c->stream));
+      HIPCHK(c, launch_adjoint_step(aa, np, ash, c->stream));
       ++launches;
     }
     s.launches = launches;
@@ -930,13 +951,22 @@ int tomo_timing_reset(tomo_ctx *c) {
   return TOMO_OK;
 }
 
-int tomo_set_tile(tomo_ctx *c, int ry) {
+int tomo_set_shape(tomo_ctx *c, int vec, int fwd_rows, int adj_rows) {
   if (!c) return TOMO_ERR_ARG;
-  if (ry != 0 && ry != 4 && ry != 8 && ry != 16) return fail(c, TOMO_ERR_ARG, "rows per thread must be 0, 4, 8 or 16");
-  c->tile_ry = ry;
+  if (!shape_valid(vec, fwd_rows, adj_rows))
+    return fail(c, TOMO_ERR_ARG, "unsupported stencil shape");
+  if (vec > 1 && c->nx % vec != 0) return fail(c, TOMO_ERR_ARG, "nx is not a multiple of the vector width");
+  c->shape_req = StencilShape{vec, fwd_rows, adj_rows, 0};
   return TOMO_OK;
 }
 
-int tomo_get_tile(const tomo_ctx *c) { return c ? tile_rows(c) : TOMO_ERR_ARG; }
+int tomo_get_shape(const tomo_ctx *c, int *vec, int *fwd_rows, int *adj_rows) {
+  if (!c || !vec || !fwd_rows || !adj_rows) return TOMO_ERR_ARG;
+  const StencilShape s = stencil_shape(c);
+  *vec = s.vec;
+  *fwd_rows = s.fwd_rows;
+  *adj_rows = s.adj_rows;
+  return TOMO_OK;
+}
 
 }  // extern "C"

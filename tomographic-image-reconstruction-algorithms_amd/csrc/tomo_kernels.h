@@ -70,10 +70,21 @@ struct AdjArgs {
   const float *sp_a, *sp_ab;
 };
 
-// Stencil launchers (grid = tiles x P shots), rows per thread ry in {4, 8, 16}.
+// Stencil kernel shape: vec = cells per lane along x (1: scalar kernels,
+// 2 or 4: vectorised kernels, FP32 only, nx % vec == 0), rows = cells per
+// lane along y (scalar: forward 4/8/16, adjoint 4/8; vectorised: forward
+// 2/4/8, adjoint 1/2/4).
+struct StencilShape {
+  int vec;
+  int fwd_rows;
+  int adj_rows;
+  int rows;  // rows of the launch at hand (set by the caller from the two above)
+};
+
+// Stencil launchers (grid = tiles x P shots).
 template <typename T>
-hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, int ry, hipStream_t st);
-hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, int ry, hipStream_t st);
+hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, const StencilShape &sh, hipStream_t st);
+hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape &sh, hipStream_t st);
 
 // pw = a(d) * (c0 dt / h)^2 * (1 + f)
 template <typename T>

@@ -44,6 +44,9 @@ __device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T
 // Leapfrog step k -> k+1 for every cell of one tile of one shot
 // (propagation, sirt-mimo.cu:511-593, and the receiver record of
 // initial_signal / difference_signal, sirt-mimo.cu:617-684, fused).
+// Each lane first issues every load of its strip (RY + 2 rows of u(k), the
+// x-neighbours, u(k-1), the coefficients, the receiver map), then computes
+// and stores: the loads of all rows are in flight together.
 template <typename T, bool SPONGE, int RY>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<T> A) {
   const int nx = A.nx, ny = A.ny;
@@ -57,24 +60,37 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<
   const T *__restrict__ um = A.um + so;
   T *__restrict__ up = A.up + so;
   const T *__restrict__ pw = A.pw;
-  const int4 bb = A.bbox[gs];
-  const bool src_col = (A.amp_h2 != T(0)) && x >= bb.x && x <= bb.y;
   const bool xin = x > 0 && x < nx - 1;
 
-  T cu = y0 > 0 ? u0[(y0 - 1) * nx + x] : T(0);
-  T cc = u0[y0 * nx + x];
+  T c[RY + 2], l[RY], r[RY], p[RY], m[RY];
+  int cm[RY];
 #pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    const int y = y0 + r;
+  for (int i = 0; i < RY + 2; ++i) {
+    const int y = y0 - 1 + i;
+    c[i] = (y >= 0 && y < ny) ? u0[y * nx + x] : T(0);
+  }
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    const bool ok = y < ny;
+    const int id = y * nx + x;
+    l[i] = (ok && xin) ? u0[id - 1] : T(0);
+    r[i] = (ok && xin) ? u0[id + 1] : T(0);
+    p[i] = ok ? pw[id] : T(0);
+    m[i] = ok ? um[id] : T(0);
+    cm[i] = (ok && A.record) ? A.cellmap[id] : -1;
+  }
+
+  const int4 bb = A.bbox[gs];
+  const bool src_col = (A.amp_h2 != T(0)) && x >= bb.x && x <= bb.y;
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
     if (y >= ny) break;
     const int id = y * nx + x;
-    const T cd = (y + 1 < ny) ? u0[id + nx] : T(0);
     T val;
     if (xin && y > 0 && y < ny - 1) {
-      const T l = u0[id - 1], rt = u0[id + 1];
-      const T p = pw[id];
-      const T sum = ((rt + l) + cu) + cd;  // u(i+1) + u(i-1) + u(j-1) + u(j+1)
-      const T prev = um[id];
+      const T sum = ((r[i] + l[i]) + c[i]) + c[i + 2];  // u(i+1) + u(i-1) + u(j-1) + u(j+1)
       if (SPONGE) {
         const int d = edge_dist(x, y, nx, ny);
         T a = T(1), ab = T(1);
@@ -82,24 +98,19 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<
           a = A.sp_a[d];
           ab = A.sp_ab[d];
         }
-        val = p * sum + (T(2) * a - T(4) * p) * cc - ab * prev;
+        val = p[i] * sum + (T(2) * a - T(4) * p[i]) * c[i + 1] - ab * m[i];
       } else {
-        val = p * sum + (T(2) - T(4) * p) * cc - prev;
+        val = p[i] * sum + (T(2) - T(4) * p[i]) * c[i + 1] - m[i];
       }
-      if (src_col && y >= bb.z && y <= bb.w) val += p * A.amp_h2;
+      if (src_col && y >= bb.z && y <= bb.w) val += p[i] * A.amp_h2;
     } else {
       val = forward_edge(A, u0, um, x, y);
     }
     up[id] = val;
-    if (A.record) {
-      const int ri = A.cellmap[id];
-      if (ri >= 0) {
-        const long long o = ((long long)gs * A.nt + A.level) * A.nr + ri;
-        A.rec[o] = A.meas ? A.meas[o] - float(val) : float(val);
-      }
+    if (cm[i] >= 0) {
+      const long long o = ((long long)gs * A.nt + A.level) * A.nr + cm[i];
+      A.rec[o] = A.meas ? A.meas[o] - float(val) : float(val);
     }
-    cu = cc;
-    cc = cd;
   }
 }
 
@@ -185,7 +196,8 @@ __device__ void adjoint_slow(const AdjArgs &A, const float *__restrict__ z1, con
 // Adjoint step k+1, k+2 -> k with the correlation acc += z(k) * Lu(k)
 // fused (backpropagation1/2 + laplace + update_differential,
 // sirt-mimo-io.cu:342-353): the adjoint field is consumed while it is in
-// registers, so z and Lu histories are never stored.
+// registers, so z and Lu histories are never stored.  Same load-then-compute
+// structure as the forward step.
 template <bool SPONGE, bool COV, int RY>
 __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_step_kernel(AdjArgs A) {
   const int nx = A.nx, ny = A.ny;
@@ -205,40 +217,296 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_step_kernel(AdjArgs 
   const float *__restrict__ pw = A.pw;
   const bool xin = x > 0 && x < nx - 1;
 
-  const int idt = y0 * nx + x;
-  float zu = y0 > 0 ? z1[idt - nx] : 0.f, zc = z1[idt];
-  float pu = y0 > 0 ? pw[idt - nx] : 0.f, pc = pw[idt];
-  float uu = y0 > 0 ? u[idt - nx] : 0.f, uc = u[idt];
+  float zc[RY + 2], pc[RY + 2], uc[RY + 2];
+  float zl[RY], zr[RY], pl[RY], pr[RY], ul[RY], ur[RY], zz[RY], ac[RY], cv[RY];
+  int cm[RY];
 #pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    const int y = y0 + r;
+  for (int i = 0; i < RY + 2; ++i) {
+    const int y = y0 - 1 + i;
+    const bool ok = y >= 0 && y < ny;
+    const int id = y * nx + x;
+    zc[i] = ok ? z1[id] : 0.f;
+    pc[i] = ok ? pw[id] : 0.f;
+    uc[i] = ok ? u[id] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    const bool ok = y < ny;
+    const bool in = ok && xin;
+    const int id = y * nx + x;
+    zl[i] = in ? z1[id - 1] : 0.f;
+    zr[i] = in ? z1[id + 1] : 0.f;
+    pl[i] = in ? pw[id - 1] : 0.f;
+    pr[i] = in ? pw[id + 1] : 0.f;
+    ul[i] = in ? u[id - 1] : 0.f;
+    ur[i] = in ? u[id + 1] : 0.f;
+    zz[i] = ok ? z2[id] : 0.f;
+    ac[i] = ok ? acc[id] : 0.f;
+    cv[i] = (ok && COV) ? cov[id] : 0.f;
+    cm[i] = (in && y > 0 && y < ny - 1) ? A.cellmap[id] : -1;
+  }
+
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
     if (y >= ny) break;
     const int id = y * nx + x;
-    const bool down = y + 1 < ny;
-    const float zd = down ? z1[id + nx] : 0.f;
-    const float pd = down ? pw[id + nx] : 0.f;
-    const float ud = down ? u[id + nx] : 0.f;
     const bool interior = xin && y > 0 && y < ny - 1;
-    const int ri = interior ? A.cellmap[id] : -1;
     float zval, lu, uval;
-    if (interior && ri < 0) {
+    if (interior && cm[i] < 0) {
       float a, ab;
       sponge_factors<SPONGE>(A, x, y, a, ab);
-      const float zl = z1[id - 1], zr = z1[id + 1];
-      const float pl = pw[id - 1], pr = pw[id + 1];
-      zval = (((pu * zu + pd * zd) + pl * zl) + pr * zr) + (2.f * a - 4.f * pc) * zc - ab * z2[id];
-      const float ul = u[id - 1], ur = u[id + 1];
-      lu = (((uu + ud) + ul) + ur - 4.f * uc) * A.inv_h2;
-      uval = uc;
+      zval = (((pc[i] * zc[i] + pc[i + 2] * zc[i + 2]) + pl[i] * zl[i]) + pr[i] * zr[i]) +
+             (2.f * a - 4.f * pc[i + 1]) * zc[i + 1] - ab * zz[i];
+      lu = (((uc[i] + uc[i + 2]) + ul[i]) + ur[i] - 4.f * uc[i + 1]) * A.inv_h2;
+      uval = uc[i + 1];
     } else {
-      adjoint_slow<SPONGE>(A, z1, z2, u, gs, x, y, ri, zval, lu, uval);
+      adjoint_slow<SPONGE>(A, z1, z2, u, gs, x, y, interior ? cm[i] : -1, zval, lu, uval);
     }
     z0[id] = zval;
-    acc[id] += zval * lu;
-    if (COV) cov[id] += uval * uval;
-    zu = zc; zc = zd;
-    pu = pc; pc = pd;
-    uu = uc; uc = ud;
+    acc[id] = ac[i] + zval * lu;
+    if (COV) cov[id] = cv[i] + uval * uval;
+  }
+}
+
+// ------------------------------------------------------ vectorised stencils
+// Same rules as the kernels above, with each lane owning VEC consecutive
+// cells of a row (one VEC-wide load per row and array instead of three
+// scalar loads): a wavefront covers 64 * VEC columns, the x-neighbours
+// crossing a lane boundary move between lanes with DPP whole-wave shifts,
+// and only the two tile-boundary lanes load their outer neighbour.
+// Requires nx % VEC == 0.  All lanes stay active through the DPP shifts.
+
+// value held by lane - 1 (lane 0 receives 0)
+__device__ __forceinline__ float from_prev_lane(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+// value held by lane + 1 (lane 63 receives 0)
+__device__ __forceinline__ float from_next_lane(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
+template <int VEC>
+struct Row {
+  float e[VEC];
+};
+
+template <int VEC>
+__device__ __forceinline__ Row<VEC> load_row(const float *__restrict__ p, bool ok) {
+  typedef float vt __attribute__((ext_vector_type(VEC)));
+  Row<VEC> r;
+  if (ok) {
+    const vt v = *reinterpret_cast<const vt *>(p);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) r.e[e] = v[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) r.e[e] = 0.f;
+  }
+  return r;
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_row(float *__restrict__ p, const Row<VEC> &r) {
+  typedef float vt __attribute__((ext_vector_type(VEC)));
+  vt v;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) v[e] = r.e[e];
+  *reinterpret_cast<vt *>(p) = v;
+}
+
+template <int VEC>
+__device__ __forceinline__ void load_row_int(const int *__restrict__ p, bool ok, int *out) {
+  typedef int vt __attribute__((ext_vector_type(VEC)));
+  if (ok) {
+    const vt v = *reinterpret_cast<const vt *>(p);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) out[e] = v[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) out[e] = -1;
+  }
+}
+
+// x-neighbours of a lane's VEC cells: left of cell 0 and right of cell VEC-1
+// (the rest are in the lane's own row).  Every lane must execute this.
+template <int VEC>
+__device__ __forceinline__ void row_ends(const Row<VEC> &r, const float *__restrict__ row_base, int x0, int nx,
+                                         bool ok, float &lft, float &rgt) {
+  lft = from_prev_lane(r.e[VEC - 1]);
+  rgt = from_next_lane(r.e[0]);
+  if (threadIdx.x == 0) lft = (ok && x0 > 0) ? row_base[x0 - 1] : 0.f;
+  if (threadIdx.x == kBlockX - 1) rgt = (ok && x0 + VEC < nx) ? row_base[x0 + VEC] : 0.f;
+}
+
+template <bool SPONGE, int VEC, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<float> A) {
+  const int nx = A.nx, ny = A.ny;
+  const int tile = blockIdx.x;
+  const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
+  const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
+  if (y0 >= ny) return;  // whole wavefront
+  const bool xok = x0 < nx;
+  const int gs = A.shot0 + blockIdx.y;
+  const long long so = (long long)blockIdx.y * A.sstride;
+  const float *__restrict__ u0 = A.u0 + so;
+  const float *__restrict__ um = A.um + so;
+  float *__restrict__ up = A.up + so;
+  const float *__restrict__ pw = A.pw;
+
+  Row<VEC> c[RY + 2], m[RY], p[RY];
+  int cm[RY][VEC];
+#pragma unroll
+  for (int i = 0; i < RY + 2; ++i) {
+    const int y = y0 - 1 + i;
+    const bool ok = xok && y >= 0 && y < ny;
+    c[i] = load_row<VEC>(u0 + (ok ? y * nx + x0 : 0), ok);
+  }
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    const bool ok = xok && y < ny;
+    const int id = ok ? y * nx + x0 : 0;
+    m[i] = load_row<VEC>(um + id, ok);
+    p[i] = load_row<VEC>(pw + id, ok);
+    load_row_int<VEC>(A.cellmap + id, ok && A.record, cm[i]);
+  }
+  float lft[RY], rgt[RY];
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    const bool ok = xok && y < ny;
+    row_ends<VEC>(c[i + 1], u0 + (ok ? y * nx : 0), x0, nx, ok, lft[i], rgt[i]);
+  }
+
+  const int4 bb = A.bbox[gs];
+  const bool src = A.amp_h2 != 0.f;
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    if (!xok || y >= ny) continue;
+    Row<VEC> out;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      const int x = x0 + e;
+      float val;
+      if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1) {
+        const float l = e == 0 ? lft[i] : c[i + 1].e[e - 1];
+        const float r = e == VEC - 1 ? rgt[i] : c[i + 1].e[e + 1];
+        const float sum = ((r + l) + c[i].e[e]) + c[i + 2].e[e];
+        const float pe = p[i].e[e];
+        if (SPONGE) {
+          const int d = edge_dist(x, y, nx, ny);
+          float a = 1.f, ab = 1.f;
+          if (d < A.sponge_w) {
+            a = A.sp_a[d];
+            ab = A.sp_ab[d];
+          }
+          val = pe * sum + (2.f * a - 4.f * pe) * c[i + 1].e[e] - ab * m[i].e[e];
+        } else {
+          val = pe * sum + (2.f - 4.f * pe) * c[i + 1].e[e] - m[i].e[e];
+        }
+        if (src && x >= bb.x && x <= bb.y && y >= bb.z && y <= bb.w) val += pe * A.amp_h2;
+      } else {
+        val = forward_edge(A, u0, um, x, y);
+      }
+      out.e[e] = val;
+      if (cm[i][e] >= 0) {
+        const long long o = ((long long)gs * A.nt + A.level) * A.nr + cm[i][e];
+        A.rec[o] = A.meas ? A.meas[o] - val : val;
+      }
+    }
+    store_row<VEC>(up + y * nx + x0, out);
+  }
+}
+
+template <bool SPONGE, bool COV, int VEC, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A) {
+  const int nx = A.nx, ny = A.ny;
+  const int tile = blockIdx.x;
+  const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
+  const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
+  if (y0 >= ny) return;  // whole wavefront
+  const bool xok = x0 < nx;
+  const int gs = A.shot0 + blockIdx.y;
+  const long long so = (long long)blockIdx.y * A.sstride;
+  const long long n2 = (long long)nx * ny;
+  const float *__restrict__ z1 = A.z1 + so;
+  const float *__restrict__ z2 = A.z2 + so;
+  const float *__restrict__ u = A.u + so;
+  float *__restrict__ z0 = A.z0 + so;
+  float *__restrict__ acc = A.acc + gs * n2;
+  float *__restrict__ cov = COV ? A.cov + gs * n2 : nullptr;
+  const float *__restrict__ pw = A.pw;
+
+  Row<VEC> zc[RY + 2], pc[RY + 2], uc[RY + 2], zz[RY], ac[RY], cv[RY];
+  int cm[RY][VEC];
+#pragma unroll
+  for (int i = 0; i < RY + 2; ++i) {
+    const int y = y0 - 1 + i;
+    const bool ok = xok && y >= 0 && y < ny;
+    const int id = ok ? y * nx + x0 : 0;
+    zc[i] = load_row<VEC>(z1 + id, ok);
+    pc[i] = load_row<VEC>(pw + id, ok);
+    uc[i] = load_row<VEC>(u + id, ok);
+  }
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    const bool ok = xok && y < ny;
+    const int id = ok ? y * nx + x0 : 0;
+    zz[i] = load_row<VEC>(z2 + id, ok);
+    ac[i] = load_row<VEC>(acc + id, ok);
+    if (COV) cv[i] = load_row<VEC>(cov + id, ok);
+    load_row_int<VEC>(A.cellmap + id, ok, cm[i]);
+  }
+  float zl[RY], zr[RY], pl[RY], pr[RY], ul[RY], ur[RY];
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    const bool ok = xok && y < ny;
+    const int rb = ok ? y * nx : 0;
+    row_ends<VEC>(zc[i + 1], z1 + rb, x0, nx, ok, zl[i], zr[i]);
+    row_ends<VEC>(pc[i + 1], pw + rb, x0, nx, ok, pl[i], pr[i]);
+    row_ends<VEC>(uc[i + 1], u + rb, x0, nx, ok, ul[i], ur[i]);
+  }
+
+#pragma unroll
+  for (int i = 0; i < RY; ++i) {
+    const int y = y0 + i;
+    if (!xok || y >= ny) continue;
+    Row<VEC> zo, ao, co;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      const int x = x0 + e;
+      const bool interior = x > 0 && x < nx - 1 && y > 0 && y < ny - 1;
+      const int ri = interior ? cm[i][e] : -1;
+      float zval, lu, uval;
+      if (interior && ri < 0) {
+        float a, ab;
+        sponge_factors<SPONGE>(A, x, y, a, ab);
+        const float zle = e == 0 ? zl[i] : zc[i + 1].e[e - 1];
+        const float zre = e == VEC - 1 ? zr[i] : zc[i + 1].e[e + 1];
+        const float ple = e == 0 ? pl[i] : pc[i + 1].e[e - 1];
+        const float pre = e == VEC - 1 ? pr[i] : pc[i + 1].e[e + 1];
+        const float ule = e == 0 ? ul[i] : uc[i + 1].e[e - 1];
+        const float ure = e == VEC - 1 ? ur[i] : uc[i + 1].e[e + 1];
+        zval = (((pc[i].e[e] * zc[i].e[e] + pc[i + 2].e[e] * zc[i + 2].e[e]) + ple * zle) + pre * zre) +
+               (2.f * a - 4.f * pc[i + 1].e[e]) * zc[i + 1].e[e] - ab * zz[i].e[e];
+        lu = (((uc[i].e[e] + uc[i + 2].e[e]) + ule) + ure - 4.f * uc[i + 1].e[e]) * A.inv_h2;
+        uval = uc[i + 1].e[e];
+      } else {
+        adjoint_slow<SPONGE>(A, z1, z2, u, gs, x, y, ri, zval, lu, uval);
+      }
+      zo.e[e] = zval;
+      ao.e[e] = ac[i].e[e] + zval * lu;
+      if (COV) co.e[e] = cv[i].e[e] + uval * uval;
+    }
+    const int id = y * nx + x0;
+    store_row<VEC>(z0 + id, zo);
+    store_row<VEC>(acc + id, ao);
+    if (COV) store_row<VEC>(cov + id, co);
   }
 }
 
@@ -421,14 +689,43 @@ hipError_t fwd_dispatch(FwdArgs<T> a, int nshots, int ry, hipStream_t st) {
   return hipGetLastError();
 }
 
+inline dim3 vec_grid(int nx, int ny, int vec, int ry, int nshots, int *tiles_x) {
+  *tiles_x = (nx + kBlockX * vec - 1) / (kBlockX * vec);
+  const int tiles_y = (ny + kBlockY * ry - 1) / (kBlockY * ry);
+  return dim3(*tiles_x * tiles_y, nshots, 1);
+}
+
+template <bool SPONGE, int VEC>
+hipError_t fwd_vec_dispatch(FwdArgs<float> a, int nshots, int ry, hipStream_t st) {
+  const dim3 block(kBlockX, kBlockY, 1);
+  const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x);
+  switch (ry) {
+    case 2: forward_vec_kernel<SPONGE, VEC, 2><<<grid, block, 0, st>>>(a); break;
+    case 8: forward_vec_kernel<SPONGE, VEC, 8><<<grid, block, 0, st>>>(a); break;
+    default: forward_vec_kernel<SPONGE, VEC, 4><<<grid, block, 0, st>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
 template <bool SPONGE, bool COV>
 hipError_t adj_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
   const dim3 block(kBlockX, kBlockY, 1);
   const dim3 grid = stencil_grid(a.nx, a.ny, ry, nshots, &a.tiles_x);
   switch (ry) {
     case 4: adjoint_step_kernel<SPONGE, COV, 4><<<grid, block, 0, st>>>(a); break;
-    case 16: adjoint_step_kernel<SPONGE, COV, 16><<<grid, block, 0, st>>>(a); break;
     default: adjoint_step_kernel<SPONGE, COV, 8><<<grid, block, 0, st>>>(a); break;
+  }
+  return hipGetLastError();
+}
+
+template <bool SPONGE, bool COV, int VEC>
+hipError_t adj_vec_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
+  const dim3 block(kBlockX, kBlockY, 1);
+  const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x);
+  switch (ry) {
+    case 1: adjoint_vec_kernel<SPONGE, COV, VEC, 1><<<grid, block, 0, st>>>(a); break;
+    case 4: adjoint_vec_kernel<SPONGE, COV, VEC, 4><<<grid, block, 0, st>>>(a); break;
+    default: adjoint_vec_kernel<SPONGE, COV, VEC, 2><<<grid, block, 0, st>>>(a); break;
   }
   return hipGetLastError();
 }
@@ -438,16 +735,44 @@ hipError_t adj_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
 // ------------------------------------------------------------- launchers
 
 template <typename T>
-hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, int ry, hipStream_t st) {
+hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, const StencilShape &sh, hipStream_t st) {
+  int ry = sh.rows;
   if (ry != 4 && ry != 8 && ry != 16) ry = 8;
   return a.sponge_w > 0 ? fwd_dispatch<T, true>(a, nshots, ry, st) : fwd_dispatch<T, false>(a, nshots, ry, st);
 }
-template hipError_t launch_forward_step<float>(const FwdArgs<float> &, int, int, hipStream_t);
-template hipError_t launch_forward_step<double>(const FwdArgs<double> &, int, int, hipStream_t);
 
-hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, int ry, hipStream_t st) {
+template <>
+hipError_t launch_forward_step<float>(const FwdArgs<float> &a, int nshots, const StencilShape &sh, hipStream_t st) {
+  const int vec = sh.vec;
+  if (vec == 2 || vec == 4) {
+    if (a.nx % vec != 0) return hipErrorInvalidValue;
+    const int ry = sh.rows == 2 || sh.rows == 8 ? sh.rows : 4;
+    const bool sp = a.sponge_w > 0;
+    if (vec == 4) return sp ? fwd_vec_dispatch<true, 4>(a, nshots, ry, st) : fwd_vec_dispatch<false, 4>(a, nshots, ry, st);
+    return sp ? fwd_vec_dispatch<true, 2>(a, nshots, ry, st) : fwd_vec_dispatch<false, 2>(a, nshots, ry, st);
+  }
+  int ry = sh.rows;
   if (ry != 4 && ry != 8 && ry != 16) ry = 8;
+  return a.sponge_w > 0 ? fwd_dispatch<float, true>(a, nshots, ry, st) : fwd_dispatch<float, false>(a, nshots, ry, st);
+}
+template hipError_t launch_forward_step<double>(const FwdArgs<double> &, int, const StencilShape &, hipStream_t);
+
+hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape &sh, hipStream_t st) {
   const bool sp = a.sponge_w > 0, cov = a.cov != nullptr;
+  const int vec = sh.vec;
+  if (vec == 2 || vec == 4) {
+    if (a.nx % vec != 0) return hipErrorInvalidValue;
+    const int ry = sh.rows == 1 || sh.rows == 4 ? sh.rows : 2;
+#define TOMO_ADJ_VEC(V)                                                                          \
+  if (sp) return cov ? adj_vec_dispatch<true, true, V>(a, nshots, ry, st) : adj_vec_dispatch<true, false, V>(a, nshots, ry, st); \
+  return cov ? adj_vec_dispatch<false, true, V>(a, nshots, ry, st) : adj_vec_dispatch<false, false, V>(a, nshots, ry, st);
+    if (vec == 4) {
+      TOMO_ADJ_VEC(4)
+    }
+    TOMO_ADJ_VEC(2)
+#undef TOMO_ADJ_VEC
+  }
+  const int ry = sh.rows <= 4 ? 4 : 8;  // the adjoint strip holds ~14 values per row in registers
   if (sp) return cov ? adj_dispatch<true, true>(a, nshots, ry, st) : adj_dispatch<true, false>(a, nshots, ry, st);
   return cov ? adj_dispatch<false, true>(a, nshots, ry, st) : adj_dispatch<false, false>(a, nshots, ry, st);
 }
