@@ -1,0 +1,78 @@
+"""The command-line drivers that mirror the reference executables: arguments,
+file names, text layout and stop rules (generate-field.cu, generate-data.cu,
+sirt-mimo-io.cu, sirt-mimo-norm-mem.cu, art-mimo-io.cu)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from _pkg import tomo
+
+BIN = os.path.join(tomo.PKG_DIR, "_build")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run(args, cwd):
+    return subprocess.run([os.path.join(BIN, args[0]), *map(str, args[1:])], cwd=cwd, capture_output=True,
+                          text=True, timeout=300)
+
+
+def test_generate_field_writes_reference_phantom(tmp_path):
+    r = run(["generate-field"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    fo = np.loadtxt(tmp_path / "fo.txt", dtype=np.float32).ravel()
+    fx = np.loadtxt(os.path.join(ROOT, "tests", "golden", "fo_192.txt"), dtype=np.float32).ravel()
+    np.testing.assert_array_equal(fo, fx)
+    text = open(tmp_path / "fo.txt").read()
+    assert text.endswith(" \n\n\n")
+
+
+def test_usage_and_filename_errors(tmp_path):
+    r = run(["sirt-mimo-io", "fo.txt"], tmp_path)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = run(["sirt-mimo-io", "a.b.txt", 10, 1, 1, 1, 1, 1, 1], tmp_path)
+    assert r.returncode == 1 and "only one period" in r.stderr
+
+
+@pytest.mark.gpu
+def test_generate_data_then_sirt_and_art(tmp_path):
+    print("[case] drivers G=160", flush=True)
+    assert run(["generate-field"], tmp_path).returncode == 0
+    r = run(["generate-data", "fo.txt", 160], tmp_path)
+    assert r.returncode == 0, r.stderr
+    for side in ("bottom", "right", "top", "left"):
+        a = np.loadtxt(tmp_path / f"fo-data-{side}-160.txt", dtype=np.float32)
+        assert a.shape == (4 * 401, 192)
+    # missing data file message (sirt-mimo-io.cu:147-150)
+    r = run(["sirt-mimo-io", "fo.txt", 32, 0, 1, 1, 1, 0, 1], tmp_path)
+    assert "fo-data-bottom-32.txt' file not found" in r.stderr
+    # two uniform SIRT iterations at a fixed scale 22500 (omega 1, alpha 22500, beta 0)
+    r = run(["sirt-mimo-io", "fo.txt", 160, 0, 2, 1, 22500, 0, 1], tmp_path)
+    assert r.returncode == 0, r.stderr
+    f_drv = np.loadtxt(tmp_path / "sirt-160-fo.txt", dtype=np.float32).ravel()
+    lines = [l for l in r.stdout.splitlines() if l.strip()]
+    assert lines[0].split() == ["iter", "epsilon", "difference"]
+    eps_drv = [float(l.split()[1]) for l in lines[2:4]]
+    assert os.path.exists(tmp_path / "sirt_convergence.txt")
+    # the same through the binding, with the driver's data files
+    with tomo.Engine(tomo.config_reference()) as e:
+        ii, jj = tomo.transducers_reference()
+        e.set_transducers(ii, jj, one_based=True)
+        ng = e.set_groups(160)
+        sides = [tomo.read_text(tmp_path / f"fo-data-{s}-160.txt", (192, 401, ng)).reshape(ng, 401, 192)
+                 for s in ("bottom", "right", "top", "left")]
+        e.set_measured(e.sides_to_traces(*sides))
+        fo = tomo.read_text(tmp_path / "fo.txt", (192, 192))
+        e.set_truth(fo)
+        eps = [e.sirt_iteration(22500.0, mode=tomo.UPDATE_MEAN)[1] for _ in range(2)]
+        np.testing.assert_allclose(e.get_field(), f_drv, rtol=1e-6, atol=1e-10)
+    np.testing.assert_allclose(eps_drv, eps, rtol=1e-6)
+    # weighted, memory-scalable passes of 3 groups; then ART
+    r = run(["sirt-mimo-norm-mem", "fo.txt", 160, 3, 0, 1, 1, 22500, 0, 1], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(tmp_path / "sirt-norm-160-fo.txt")
+    r = run(["art-mimo-io", "fo.txt", 160, 0, 1, 1, 20000, 0, 1], tmp_path)
+    assert r.returncode == 0, r.stderr
+    f_art = np.loadtxt(tmp_path / "art-160-fo.txt", dtype=np.float32)
+    assert np.isfinite(f_art).all() and np.abs(f_art).max() > 0

@@ -38,7 +38,7 @@ struct FwdArgs {
   const float *meas;    // measured traces [S][nt][nr] or null
   float *rec;           // traces / residuals [S][nt][nr] or null
   int nx, ny, nt, nr;
-  int tiles_x;
+  int tiles_x, tiles_per_shot;
   int shot0;            // global index of local shot 0
   int level;            // k + 1
   int record;           // record this level
@@ -61,7 +61,7 @@ struct AdjArgs {
   float *acc;           // gradient accumulators [S][n2] (global shot index)
   float *cov;           // coverage accumulators [S][n2] or null
   int nx, ny, nt, nr;
-  int tiles_x;
+  int tiles_x, tiles_per_shot;
   int shot0;
   int level;            // k
   float gain;           // residual injection factor

@@ -287,6 +287,18 @@ __device__ __forceinline__ float from_next_lane(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map the physical block id so that every XCD owns one contiguous range
+// of logical blocks: neighbouring tiles of a shot, which share halo rows,
+// then run on the same XCD.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_logical_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int xcd = b % kXcds, loc = b / kXcds;
+  return xcd * per + min(xcd, rem) + loc;
+}
+
 template <int VEC>
 struct Row {
   float e[VEC];
@@ -343,13 +355,14 @@ __device__ __forceinline__ void row_ends(const Row<VEC> &r, const float *__restr
 template <bool SPONGE, int VEC, int RY>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<float> A) {
   const int nx = A.nx, ny = A.ny;
-  const int tile = blockIdx.x;
+  const int lb = xcd_logical_block();
+  const int tile = lb % A.tiles_per_shot, shot = lb / A.tiles_per_shot;
   const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
   const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
   if (y0 >= ny) return;  // whole wavefront
   const bool xok = x0 < nx;
-  const int gs = A.shot0 + blockIdx.y;
-  const long long so = (long long)blockIdx.y * A.sstride;
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
   const float *__restrict__ u0 = A.u0 + so;
   const float *__restrict__ um = A.um + so;
   float *__restrict__ up = A.up + so;
@@ -424,13 +437,14 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<f
 template <bool SPONGE, bool COV, int VEC, int RY>
 __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A) {
   const int nx = A.nx, ny = A.ny;
-  const int tile = blockIdx.x;
+  const int lb = xcd_logical_block();
+  const int tile = lb % A.tiles_per_shot, shot = lb / A.tiles_per_shot;
   const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
   const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
   if (y0 >= ny) return;  // whole wavefront
   const bool xok = x0 < nx;
-  const int gs = A.shot0 + blockIdx.y;
-  const long long so = (long long)blockIdx.y * A.sstride;
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
   const long long n2 = (long long)nx * ny;
   const float *__restrict__ z1 = A.z1 + so;
   const float *__restrict__ z2 = A.z2 + so;
@@ -689,16 +703,18 @@ hipError_t fwd_dispatch(FwdArgs<T> a, int nshots, int ry, hipStream_t st) {
   return hipGetLastError();
 }
 
-inline dim3 vec_grid(int nx, int ny, int vec, int ry, int nshots, int *tiles_x) {
+// 1-D grid of tiles x shots (the kernels remap it per XCD)
+inline dim3 vec_grid(int nx, int ny, int vec, int ry, int nshots, int *tiles_x, int *tiles_per_shot) {
   *tiles_x = (nx + kBlockX * vec - 1) / (kBlockX * vec);
   const int tiles_y = (ny + kBlockY * ry - 1) / (kBlockY * ry);
-  return dim3(*tiles_x * tiles_y, nshots, 1);
+  *tiles_per_shot = *tiles_x * tiles_y;
+  return dim3(*tiles_per_shot * nshots, 1, 1);
 }
 
 template <bool SPONGE, int VEC>
 hipError_t fwd_vec_dispatch(FwdArgs<float> a, int nshots, int ry, hipStream_t st) {
   const dim3 block(kBlockX, kBlockY, 1);
-  const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x);
+  const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x, &a.tiles_per_shot);
   switch (ry) {
     case 2: forward_vec_kernel<SPONGE, VEC, 2><<<grid, block, 0, st>>>(a); break;
     case 8: forward_vec_kernel<SPONGE, VEC, 8><<<grid, block, 0, st>>>(a); break;
@@ -721,7 +737,7 @@ hipError_t adj_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
 template <bool SPONGE, bool COV, int VEC>
 hipError_t adj_vec_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
   const dim3 block(kBlockX, kBlockY, 1);
-  const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x);
+  const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x, &a.tiles_per_shot);
   switch (ry) {
     case 1: adjoint_vec_kernel<SPONGE, COV, VEC, 1><<<grid, block, 0, st>>>(a); break;
     case 4: adjoint_vec_kernel<SPONGE, COV, VEC, 4><<<grid, block, 0, st>>>(a); break;
