@@ -13,7 +13,7 @@ checkpointed every 200 steps, smooth random velocity (Gaussian-filtered noise,
 sigma 8 cells, +-8 %) plus 4 sharp inclusions; measured traces from the FP64
 solver.
 
-Also reported: the roofline of the forward stencil kernel (algorithmic 16 B
+Also reported: the roofline of the forward stencil kernel (algorithmic 12 B
 per cell-update, average launch time from HIP events on the engine's
 stream), the adjoint kernel's figures, and the CPU baseline (the oracle's FP32
 transcription, OpenMP over shots) on a bounded sample of the same workload.
@@ -36,8 +36,11 @@ for p in (os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")):
 from _pkg import tomo  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-FWD_BYTES = 16.0   # read u(k), u(k-1), coefficient; write u(k+1)
-ADJ_BYTES = 20.0   # read z(k+1), z(k+2), u(k), coefficient; write z(k)
+# Algorithmic HBM bytes per cell-update of one shot.  The per-cell
+# coefficient is one image shared by every shot of a launch, so it adds
+# 4/shots bytes (cached) and is not counted.
+FWD_BYTES = 12.0   # read u(k), u(k-1); write u(k+1)
+ADJ_BYTES = 24.0   # read z(k+1), z(k+2), u(k), gradient; write z(k), gradient
 
 WORKLOADS = {
     # name: (n, steps, shots, checkpoint)
@@ -83,7 +86,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
-    ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows (default: engine choice)")
+    ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows,fwd_depth,adj_depth (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     args = ap.parse_args()
@@ -157,14 +160,14 @@ def main():
             "eps_after_steps": eps[-1] if eps else None,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "forward_step_kernel",
+            "bound": "hbm", "kernel": "forward_vec_kernel",
             "achieved": round(fwd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_cell_update": FWD_BYTES, "cells_per_launch": cells_per_launch,
             "avg_launch_ms": round(fwd_avg_ms, 5),
         },
         "adjoint": {
-            "kernel": "adjoint_step_kernel", "achieved_GBs": round(adj_gbs, 1),
+            "kernel": "adjoint_vec_kernel", "achieved_GBs": round(adj_gbs, 1),
             "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "bytes_per_cell_update": ADJ_BYTES,
             "avg_launch_ms": round(adj_avg_ms, 5),
         },

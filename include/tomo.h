@@ -294,13 +294,16 @@ int tomo_timing(tomo_ctx *ctx, double *out6);
 int tomo_timing_reset(tomo_ctx *ctx);
 
 /* Shape of the stencil kernels: vec = cells per lane along x (1 scalar, 2 or
- * 4 vectorised; FP32 solver only, nx % vec == 0), rows per lane along y for
- * the forward step (vec 1: 4/8/16; vec 2/4: 2/4/8) and the adjoint step
- * (vec 1: 4/8; vec 2/4: 1/2/4).  0 in a field picks the default.  Every shape
- * gives the same results up to FP32 rounding.  No reference counterpart (its
- * launch shapes are fixed, sirt-mimo.cu:165-299). */
-int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows);
-int tomo_get_shape(const tomo_ctx *ctx, int *vec, int *fwd_rows, int *adj_rows);
+ * 4 vectorised; FP32 solver only, nx % vec == 0); rows per lane along y for
+ * the forward step (vec 1: 4/8/16; vec 2/4: 4/8/16/32) and the adjoint step
+ * (vec 1: 4/8; vec 2/4: 4/8/16/32); prefetch depth of the vectorised kernels
+ * (rows whose loads are in flight ahead of the row being computed: 1, 2, 4).
+ * 0 in a field picks the default.  Every shape gives the same results up to
+ * FP32 rounding.  No reference counterpart (its launch shapes are fixed,
+ * sirt-mimo.cu:165-299). */
+int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth);
+/* out5 = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth} in effect */
+int tomo_get_shape(const tomo_ctx *ctx, int *out5);
 
 #ifdef __cplusplus
 }

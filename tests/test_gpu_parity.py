@@ -170,7 +170,9 @@ def test_checkpoint_and_shot_passes_are_bitwise_identical():
         np.testing.assert_array_equal(G, results[0][1])
 
 
-SHAPES = [(1, 4, 4), (1, 8, 8), (1, 16, 4), (2, 2, 1), (2, 4, 2), (2, 8, 4), (4, 2, 1), (4, 4, 2), (4, 8, 4)]
+# (vec, forward rows, adjoint rows, forward prefetch depth, adjoint prefetch depth)
+SHAPES = [(1, 4, 4, 1, 1), (1, 8, 8, 1, 1), (1, 16, 4, 1, 1), (2, 4, 4, 1, 1), (2, 16, 8, 2, 4),
+          (2, 32, 32, 4, 2), (4, 4, 8, 4, 4), (4, 16, 16, 4, 2), (4, 32, 4, 2, 1)]
 
 
 @pytest.mark.parametrize("n", [64, 192])
@@ -187,18 +189,19 @@ def test_stencil_shapes_agree(n):
     meas = orc.forward(c, fo, bbox, rc, np.float64)
     Gw, _ = orc.gradient(c, np.zeros(fo.size), bbox, rc, ra, meas, np.float64)
     by_vec = {}
-    for vec, fr, ar in SHAPES:
+    for shape in SHAPES:
+        vec = shape[0]
         e, c2, _ = ref_engine(n, nt, G, field=fo)
-        e.set_shape(vec, fr, ar)
-        assert e.get_shape() == (vec, fr, ar)
+        e.set_shape(*shape)
+        assert e.get_shape() == shape
         tr = e.forward()
         e.set_field(np.zeros(fo.size, np.float32))
         e.set_measured(meas.astype(np.float32))
         e.gradient()
         Gg = np.stack([e.get_gradient(g) for g in range(e.num_shots)])
         e.close()
-        assert rel_l2(tr, meas) < TR32, (vec, fr, ar)
-        assert rel_l2(Gg.sum(0), Gw.sum(0)) < G_SUM, (vec, fr, ar)
+        assert rel_l2(tr, meas) < TR32, shape
+        assert rel_l2(Gg.sum(0), Gw.sum(0)) < G_SUM, shape
         if vec in by_vec:
             np.testing.assert_array_equal(tr, by_vec[vec][0])
             np.testing.assert_array_equal(Gg, by_vec[vec][1])
@@ -213,8 +216,8 @@ def test_vectorised_unit_preset():
     rc, ra = unit_receivers(c, ii, jj)
     meas = orc.forward(c, fo, bbox, rc, np.float64)
     Gw, _ = orc.gradient(c, np.zeros(fo.size), bbox, rc, ra, meas, np.float64)
-    for vec, fr, ar in [(2, 4, 2), (4, 4, 1)]:
-        e.set_shape(vec, fr, ar)
+    for shape in [(2, 8, 4, 2, 2), (4, 16, 16, 4, 2)]:
+        e.set_shape(*shape)
         e.set_field(fo)
         assert rel_l2(e.forward(), meas) < TR32
         e.set_field(np.zeros(fo.size, np.float32))

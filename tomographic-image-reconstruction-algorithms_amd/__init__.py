@@ -118,8 +118,8 @@ SIGNATURES = {
     "tomo_read_text": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int, _int]),
     "tomo_timing": (_int, [_vp, _f64p]),
     "tomo_timing_reset": (_int, [_vp]),
-    "tomo_set_shape": (_int, [_vp, _int, _int, _int]),
-    "tomo_get_shape": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "tomo_set_shape": (_int, [_vp, _int, _int, _int, _int, _int]),
+    "tomo_get_shape": (_int, [_vp, _i32p]),
 }
 
 
@@ -385,10 +385,11 @@ class Engine:
     def timing_reset(self):
         self._c(lib().tomo_timing_reset(self._ctx))
 
-    def set_shape(self, vec=0, fwd_rows=0, adj_rows=0):
-        self._c(lib().tomo_set_shape(self._ctx, int(vec), int(fwd_rows), int(adj_rows)))
+    def set_shape(self, vec=0, fwd_rows=0, adj_rows=0, fwd_depth=0, adj_depth=0):
+        self._c(lib().tomo_set_shape(self._ctx, int(vec), int(fwd_rows), int(adj_rows), int(fwd_depth),
+                                     int(adj_depth)))
 
     def get_shape(self):
-        v, f, a = _int(0), _int(0), _int(0)
-        self._c(lib().tomo_get_shape(self._ctx, ctypes.byref(v), ctypes.byref(f), ctypes.byref(a)))
-        return v.value, f.value, a.value
+        out = np.zeros(5, np.int32)
+        self._c(lib().tomo_get_shape(self._ctx, out))
+        return tuple(int(v) for v in out)

@@ -49,6 +49,9 @@ struct tomo_ctx {
   double *d_pw64 = nullptr;
   bool pw64_valid = false;
   int *d_cellmap = nullptr, *d_anchor = nullptr;
+  int2 *d_ring = nullptr;                 // edge cells + receiver cells for the vectorised kernels
+  int nring = 0;
+  unsigned char *d_rowflag = nullptr;     // per row: holds a receiver
   int4 *d_bbox = nullptr;
   float *d_spa = nullptr, *d_spab = nullptr;
   double *d_spa64 = nullptr, *d_spab64 = nullptr;
@@ -66,7 +69,7 @@ struct tomo_ctx {
   std::vector<char> grad_valid;
   std::vector<float> amp32;   // pulse per step, float arithmetic
   std::vector<double> amp64;  // pulse per step, double arithmetic
-  StencilShape shape_req{0, 0, 0, 0};  // requested (0 = automatic)
+  StencilShape shape_req{};  // requested fields (0 = automatic)
   // timing
   std::vector<hipEvent_t> ev_free;
   std::vector<TimedSpan> spans;
@@ -126,17 +129,29 @@ StencilShape stencil_shape(const tomo_ctx *c) {
   // 4-wide lanes when a 256-column tile is filled (or nearly), else 2-wide
   if (s.vec == 0) s.vec = (c->nx % 4 == 0 && (c->nx % 256 == 0 || c->nx > 256)) ? 4 : (c->nx % 2 == 0 ? 2 : 1);
   if (c->nx % s.vec != 0) s.vec = 1;
-  // measured best on 512 x 512 x 128 shots: 2 rows per lane forward, 1 adjoint
-  s.fwd_rows = c->shape_req.fwd_rows ? c->shape_req.fwd_rows : (s.vec > 1 ? 2 : 4);
-  s.adj_rows = c->shape_req.adj_rows ? c->shape_req.adj_rows : (s.vec > 1 ? 1 : 4);
+  // vectorised kernels stream a strip of rows per lane; taller strips cut
+  // the halo re-read (2/rows) but leave fewer strips to fill the GPU.
+  // Measured at 512^2 x 128 shots: forward 32 rows with 2 rows of loads in
+  // flight, adjoint 16 rows with 1 (its deeper queues cost occupancy).
+  s.fwd_rows = c->shape_req.fwd_rows ? c->shape_req.fwd_rows
+                                     : (s.vec > 1 ? (c->ny >= 512 ? 32 : c->ny >= 256 ? 16 : 8) : 4);
+  s.adj_rows = c->shape_req.adj_rows ? c->shape_req.adj_rows : (s.vec > 1 ? (c->ny >= 256 ? 16 : 8) : 4);
+  s.fwd_depth = c->shape_req.fwd_depth ? c->shape_req.fwd_depth : 2;
+  s.adj_depth = c->shape_req.adj_depth ? c->shape_req.adj_depth : 1;
   s.rows = s.fwd_rows;
+  s.depth = s.fwd_depth;
+  s.ring = c->d_ring;
+  s.nring = c->nring;
   return s;
 }
+
+bool depth_valid(int d) { return d == 0 || d == 1 || d == 2 || d == 4; }
 
 bool shape_valid(int vec, int fr, int ar) {
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4) return false;
   if (vec == 1) return (fr == 0 || fr == 4 || fr == 8 || fr == 16) && (ar == 0 || ar == 4 || ar == 8);
-  return (fr == 0 || fr == 2 || fr == 4 || fr == 8) && (ar == 0 || ar == 1 || ar == 2 || ar == 4);
+  auto ok = [](int r) { return r == 0 || r == 4 || r == 8 || r == 16 || r == 32; };
+  return ok(fr) && ok(ar);
 }
 
 hipEvent_t take_event(tomo_ctx *c) {
@@ -233,8 +248,27 @@ int upload_receivers(tomo_ctx *c) {
   std::vector<int> map((size_t)c->n2, -1);
   for (size_t r = 0; r < c->rcell.size(); ++r) map[c->rcell[r]] = (int)r;
   HIPCHK(c, hipMemcpyAsync(c->d_cellmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, c->stream));
-  int rc = dalloc(c, &c->d_anchor, std::max<size_t>(c->ranchor.size(), 1));
+  // cells of the edge/receiver pass: the perimeter, then the receivers
+  std::vector<int2> ring;
+  for (int x = 0; x < c->nx; ++x) ring.push_back(make_int2(x, -1));
+  for (int y = 1; y < c->ny - 1; ++y) {
+    ring.push_back(make_int2(y * c->nx, -1));
+    ring.push_back(make_int2(y * c->nx + c->nx - 1, -1));
+  }
+  for (int x = 0; x < c->nx; ++x) ring.push_back(make_int2((c->ny - 1) * c->nx + x, -1));
+  std::vector<unsigned char> rowflag(c->ny, 0);
+  for (size_t r = 0; r < c->rcell.size(); ++r) {
+    ring.push_back(make_int2(c->rcell[r], (int)r));
+    rowflag[c->rcell[r] / c->nx] = 1;
+  }
+  int rc = dalloc(c, &c->d_ring, ring.size());
   if (rc) return rc;
+  if ((rc = dalloc(c, &c->d_rowflag, (size_t)c->ny))) return rc;
+  c->nring = (int)ring.size();
+  HIPCHK(c, hipMemcpyAsync(c->d_ring, ring.data(), sizeof(int2) * ring.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_rowflag, rowflag.data(), c->ny, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((rc = dalloc(c, &c->d_anchor, std::max<size_t>(c->ranchor.size(), 1)))) return rc;
   if (!c->ranchor.empty())
     HIPCHK(c, hipMemcpyAsync(c->d_anchor, c->ranchor.data(), sizeof(int) * c->ranchor.size(),
                              hipMemcpyHostToDevice, c->stream));
@@ -420,6 +454,7 @@ AdjArgs adj_args(tomo_ctx *c, int shot0) {
   a.sponge_w = c->cfg.sponge_width;
   a.sp_a = c->d_spa;
   a.sp_ab = c->d_spab;
+  a.row_has_recv = c->d_rowflag;
   return a;
 }
 
@@ -458,6 +493,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   AdjArgs aa = adj_args(c, p0);
   StencilShape ash = stencil_shape(c);
   ash.rows = ash.adj_rows;
+  ash.depth = ash.adj_depth;
   for (int m = nseg - 1; m >= 0; --m) {
     const int k0 = m * C;
     const int cnt = std::min(C, nt - k0);
@@ -604,7 +640,7 @@ void tomo_destroy(tomo_ctx *c) {
   }
   for (auto e : c->ev_free) hipEventDestroy(e);
   dfree(c->d_f); dfree(c->d_fo); dfree(c->d_pw); dfree(c->d_pw64);
-  dfree(c->d_cellmap); dfree(c->d_anchor); dfree(c->d_bbox);
+  dfree(c->d_cellmap); dfree(c->d_anchor); dfree(c->d_bbox); dfree(c->d_ring); dfree(c->d_rowflag);
   dfree(c->d_spa); dfree(c->d_spab); dfree(c->d_spa64); dfree(c->d_spab64);
   dfree(c->d_meas); dfree(c->d_rr); dfree(c->d_grad); dfree(c->d_cov); dfree(c->d_misfit);
   dfree(c->d_scratch); dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_z); dfree(c->d_ring64);
@@ -951,21 +987,28 @@ int tomo_timing_reset(tomo_ctx *c) {
   return TOMO_OK;
 }
 
-int tomo_set_shape(tomo_ctx *c, int vec, int fwd_rows, int adj_rows) {
+int tomo_set_shape(tomo_ctx *c, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth) {
   if (!c) return TOMO_ERR_ARG;
-  if (!shape_valid(vec, fwd_rows, adj_rows))
+  if (!shape_valid(vec, fwd_rows, adj_rows) || !depth_valid(fwd_depth) || !depth_valid(adj_depth))
     return fail(c, TOMO_ERR_ARG, "unsupported stencil shape");
   if (vec > 1 && c->nx % vec != 0) return fail(c, TOMO_ERR_ARG, "nx is not a multiple of the vector width");
-  c->shape_req = StencilShape{vec, fwd_rows, adj_rows, 0};
+  c->shape_req = StencilShape{};
+  c->shape_req.vec = vec;
+  c->shape_req.fwd_rows = fwd_rows;
+  c->shape_req.adj_rows = adj_rows;
+  c->shape_req.fwd_depth = fwd_depth;
+  c->shape_req.adj_depth = adj_depth;
   return TOMO_OK;
 }
 
-int tomo_get_shape(const tomo_ctx *c, int *vec, int *fwd_rows, int *adj_rows) {
-  if (!c || !vec || !fwd_rows || !adj_rows) return TOMO_ERR_ARG;
+int tomo_get_shape(const tomo_ctx *c, int *out5) {
+  if (!c || !out5) return TOMO_ERR_ARG;
   const StencilShape s = stencil_shape(c);
-  *vec = s.vec;
-  *fwd_rows = s.fwd_rows;
-  *adj_rows = s.adj_rows;
+  out5[0] = s.vec;
+  out5[1] = s.fwd_rows;
+  out5[2] = s.adj_rows;
+  out5[3] = s.fwd_depth;
+  out5[4] = s.adj_depth;
   return TOMO_OK;
 }
 

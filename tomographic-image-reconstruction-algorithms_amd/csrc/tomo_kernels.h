@@ -66,19 +66,27 @@ struct AdjArgs {
   int level;            // k
   float gain;           // residual injection factor
   float inv_h2;
+  const unsigned char *row_has_recv;  // per row: 1 if the row holds a receiver
   int sponge_w;
   const float *sp_a, *sp_ab;
 };
 
 // Stencil kernel shape: vec = cells per lane along x (1: scalar kernels,
 // 2 or 4: vectorised kernels, FP32 only, nx % vec == 0), rows = cells per
-// lane along y (scalar: forward 4/8/16, adjoint 4/8; vectorised: forward
-// 2/4/8, adjoint 1/2/4).
+// lane along y (scalar: forward 4/8/16, adjoint 4/8; vectorised: 4/8/16/32
+// for both).
 struct StencilShape {
   int vec;
   int fwd_rows;
   int adj_rows;
-  int rows;  // rows of the launch at hand (set by the caller from the two above)
+  int fwd_depth;  // vectorised kernels: rows whose loads are in flight ahead (1, 2, 4)
+  int adj_depth;
+  int rows;   // rows of the launch at hand (set by the caller from the fields above)
+  int depth;  // prefetch depth of the launch at hand
+  // vectorised kernels: cells done by the edge/receiver pass, {flat cell,
+  // receiver index or -1} (edges and receivers of one shot)
+  const int2 *ring;
+  int nring;
 };
 
 // Stencil launchers (grid = tiles x P shots).

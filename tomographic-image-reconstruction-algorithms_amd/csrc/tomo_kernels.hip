@@ -352,7 +352,27 @@ __device__ __forceinline__ void row_ends(const Row<VEC> &r, const float *__restr
   if (threadIdx.x == kBlockX - 1) rgt = (ok && x0 + VEC < nx) ? row_base[x0 + VEC] : 0.f;
 }
 
-template <bool SPONGE, int VEC, int RY>
+// Forward step, vectorised: each lane walks a strip of RY rows as a sliding
+// window (rows above / at / below in registers) and issues the loads of the
+// next row before computing the current one, so a wavefront streams every
+// input row once (halo overhead (RY + 2) / RY) with few live registers.
+// Interior cells only: the edge cells and the receiver record are done by
+// forward_ring_kernel right after, so the hot loop has no cell map and no
+// slow path.
+template <int VEC>
+__device__ __forceinline__ void store_interior(float *__restrict__ p, const Row<VEC> &r, int x0, int nx) {
+  if (x0 > 0 && x0 + VEC < nx) {
+    store_row<VEC>(p, r);
+  } else {
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      const int x = x0 + e;
+      if (x > 0 && x < nx - 1) p[e] = r.e[e];
+    }
+  }
+}
+
+template <bool SPONGE, int VEC, int RY, int PD>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<float> A) {
   const int nx = A.nx, ny = A.ny;
   const int lb = xcd_logical_block();
@@ -367,74 +387,114 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<f
   const float *__restrict__ um = A.um + so;
   float *__restrict__ up = A.up + so;
   const float *__restrict__ pw = A.pw;
-
-  Row<VEC> c[RY + 2], m[RY], p[RY];
-  int cm[RY][VEC];
-#pragma unroll
-  for (int i = 0; i < RY + 2; ++i) {
-    const int y = y0 - 1 + i;
-    const bool ok = xok && y >= 0 && y < ny;
-    c[i] = load_row<VEC>(u0 + (ok ? y * nx + x0 : 0), ok);
-  }
-#pragma unroll
-  for (int i = 0; i < RY; ++i) {
-    const int y = y0 + i;
-    const bool ok = xok && y < ny;
-    const int id = ok ? y * nx + x0 : 0;
-    m[i] = load_row<VEC>(um + id, ok);
-    p[i] = load_row<VEC>(pw + id, ok);
-    load_row_int<VEC>(A.cellmap + id, ok && A.record, cm[i]);
-  }
-  float lft[RY], rgt[RY];
-#pragma unroll
-  for (int i = 0; i < RY; ++i) {
-    const int y = y0 + i;
-    const bool ok = xok && y < ny;
-    row_ends<VEC>(c[i + 1], u0 + (ok ? y * nx : 0), x0, nx, ok, lft[i], rgt[i]);
-  }
-
   const int4 bb = A.bbox[gs];
   const bool src = A.amp_h2 != 0.f;
+  const int yend = min(y0 + RY, ny);  // rows of this strip: [y0, yend)
+
+  // u(k) rows y0-1 .. yend are needed; u(k-1) and the coefficients rows y0 .. yend-1
+  auto uok = [&](int y) { return xok && y >= y0 - 1 && y <= yend && y >= 0 && y < ny; };
+  auto sok = [&](int y) { return xok && y >= y0 && y < yend; };
+  auto at = [&](int y, bool ok) { return ok ? y * nx + x0 : 0; };
+
+  // queue of the next PD rows: qu[j] = u(k) row y+1+j, qm/qp = u(k-1) / pw row y+j
+  Row<VEC> cu = load_row<VEC>(u0 + at(y0 - 1, uok(y0 - 1)), uok(y0 - 1));
+  Row<VEC> cc = load_row<VEC>(u0 + at(y0, uok(y0)), uok(y0));
+  Row<VEC> qu[PD], qm[PD], qp[PD];
 #pragma unroll
-  for (int i = 0; i < RY; ++i) {
-    const int y = y0 + i;
-    if (!xok || y >= ny) continue;
-    Row<VEC> out;
+  for (int j = 0; j < PD; ++j) {
+    qu[j] = load_row<VEC>(u0 + at(y0 + 1 + j, uok(y0 + 1 + j)), uok(y0 + 1 + j));
+    qm[j] = load_row<VEC>(um + at(y0 + j, sok(y0 + j)), sok(y0 + j));
+    qp[j] = load_row<VEC>(pw + at(y0 + j, sok(y0 + j)), sok(y0 + j));
+  }
+
+#pragma unroll 1
+  for (int base = y0; base < yend; base += PD) {
+    // loads of the next PD rows are in flight while these PD rows compute
+    Row<VEC> nu[PD], nm[PD], np[PD];
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-      const int x = x0 + e;
-      float val;
-      if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1) {
-        const float l = e == 0 ? lft[i] : c[i + 1].e[e - 1];
-        const float r = e == VEC - 1 ? rgt[i] : c[i + 1].e[e + 1];
-        const float sum = ((r + l) + c[i].e[e]) + c[i + 2].e[e];
-        const float pe = p[i].e[e];
-        if (SPONGE) {
-          const int d = edge_dist(x, y, nx, ny);
-          float a = 1.f, ab = 1.f;
-          if (d < A.sponge_w) {
-            a = A.sp_a[d];
-            ab = A.sp_ab[d];
-          }
-          val = pe * sum + (2.f * a - 4.f * pe) * c[i + 1].e[e] - ab * m[i].e[e];
-        } else {
-          val = pe * sum + (2.f - 4.f * pe) * c[i + 1].e[e] - m[i].e[e];
-        }
-        if (src && x >= bb.x && x <= bb.y && y >= bb.z && y <= bb.w) val += pe * A.amp_h2;
-      } else {
-        val = forward_edge(A, u0, um, x, y);
-      }
-      out.e[e] = val;
-      if (cm[i][e] >= 0) {
-        const long long o = ((long long)gs * A.nt + A.level) * A.nr + cm[i][e];
-        A.rec[o] = A.meas ? A.meas[o] - val : val;
-      }
+    for (int j = 0; j < PD; ++j) {
+      const int yu = base + PD + 1 + j, ys = base + PD + j;
+      nu[j] = load_row<VEC>(u0 + at(yu, uok(yu)), uok(yu));
+      nm[j] = load_row<VEC>(um + at(ys, sok(ys)), sok(ys));
+      np[j] = load_row<VEC>(pw + at(ys, sok(ys)), sok(ys));
     }
-    store_row<VEC>(up + y * nx + x0, out);
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      const int y = base + j;
+      if (y < yend) {  // wavefront-uniform
+        float lft, rgt;
+        row_ends<VEC>(cc, u0 + (xok ? y * nx : 0), x0, nx, xok, lft, rgt);
+        if (xok && y > 0 && y < ny - 1) {
+          Row<VEC> out;
+          const bool src_row = src && y >= bb.z && y <= bb.w;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const int x = x0 + e;
+            const float l = e == 0 ? lft : cc.e[e - 1];
+            const float r = e == VEC - 1 ? rgt : cc.e[e + 1];
+            const float sum = ((r + l) + cu.e[e]) + qu[j].e[e];
+            const float pe = qp[j].e[e];
+            float val;
+            if (SPONGE) {
+              const int d = edge_dist(x, y, nx, ny);
+              float a = 1.f, ab = 1.f;
+              if (d < A.sponge_w) {
+                a = A.sp_a[d];
+                ab = A.sp_ab[d];
+              }
+              val = pe * sum + (2.f * a - 4.f * pe) * cc.e[e] - ab * qm[j].e[e];
+            } else {
+              val = pe * sum + (2.f - 4.f * pe) * cc.e[e] - qm[j].e[e];
+            }
+            if (src_row && x >= bb.x && x <= bb.y) val += pe * A.amp_h2;
+            out.e[e] = val;
+          }
+          store_interior<VEC>(up + y * nx + x0, out, x0, nx);
+        }
+      }
+      cu = cc;
+      cc = qu[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      qu[j] = nu[j];
+      qm[j] = nm[j];
+      qp[j] = np[j];
+    }
   }
 }
 
-template <bool SPONGE, bool COV, int VEC, int RY>
+// Edge cells and receiver record of one forward step, for the vectorised
+// kernels (one thread per listed cell and shot; runs after forward_vec_kernel
+// on the same stream).  cells[i] = {flat cell, receiver index or -1}.
+template <bool SPONGE>
+__global__ __launch_bounds__(256) void forward_ring_kernel(FwdArgs<float> A, const int2 *__restrict__ cells,
+                                                           int ncells) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  const int2 c = cells[i];
+  const int gs = A.shot0 + blockIdx.y;
+  const long long so = (long long)blockIdx.y * A.sstride;
+  const int x = c.x % A.nx, y = c.x / A.nx;
+  float val;
+  if (c.y < 0) {
+    val = forward_edge(A, A.u0 + so, A.um + so, x, y);
+    A.up[so + c.x] = val;
+  } else {
+    val = A.up[so + c.x];
+  }
+  if (A.record && c.y >= 0) {
+    const long long o = ((long long)gs * A.nt + A.level) * A.nr + c.y;
+    A.rec[o] = A.meas ? A.meas[o] - val : val;
+  }
+}
+
+// Adjoint step, vectorised, same sliding-window structure: z(k+1), the
+// coefficients and u(k) stream through a three-row window; z(k+2) and the
+// accumulators are read for the current row only.  Interior cells that are
+// not receivers only (rows holding receivers are flagged per row; their
+// receiver cells are skipped): adjoint_ring_kernel does the rest.
+template <bool SPONGE, bool COV, int VEC, int RY, int PD>
 __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A) {
   const int nx = A.nx, ny = A.ny;
   const int lb = xcd_logical_block();
@@ -453,75 +513,136 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A
   float *__restrict__ acc = A.acc + gs * n2;
   float *__restrict__ cov = COV ? A.cov + gs * n2 : nullptr;
   const float *__restrict__ pw = A.pw;
+  const int yend = min(y0 + RY, ny);
 
-  Row<VEC> zc[RY + 2], pc[RY + 2], uc[RY + 2], zz[RY], ac[RY], cv[RY];
-  int cm[RY][VEC];
+  auto wok = [&](int y) { return xok && y >= y0 - 1 && y <= yend && y >= 0 && y < ny; };  // window rows
+  auto sok = [&](int y) { return xok && y >= y0 && y < yend; };                            // strip rows
+  auto at = [&](int y, bool ok) { return ok ? y * nx + x0 : 0; };
+
+  // window rows (z(k+1), pw, u(k)) above / at the current row, queue of the
+  // next PD rows below, and the strip-only rows (z(k+2), accumulators)
+  Row<VEC> zu = load_row<VEC>(z1 + at(y0 - 1, wok(y0 - 1)), wok(y0 - 1));
+  Row<VEC> pu = load_row<VEC>(pw + at(y0 - 1, wok(y0 - 1)), wok(y0 - 1));
+  Row<VEC> uu = load_row<VEC>(u + at(y0 - 1, wok(y0 - 1)), wok(y0 - 1));
+  Row<VEC> zc = load_row<VEC>(z1 + at(y0, wok(y0)), wok(y0));
+  Row<VEC> pc = load_row<VEC>(pw + at(y0, wok(y0)), wok(y0));
+  Row<VEC> uc = load_row<VEC>(u + at(y0, wok(y0)), wok(y0));
+  Row<VEC> qz[PD], qp[PD], qu[PD], q2[PD], qa[PD], qc[PD];
 #pragma unroll
-  for (int i = 0; i < RY + 2; ++i) {
-    const int y = y0 - 1 + i;
-    const bool ok = xok && y >= 0 && y < ny;
-    const int id = ok ? y * nx + x0 : 0;
-    zc[i] = load_row<VEC>(z1 + id, ok);
-    pc[i] = load_row<VEC>(pw + id, ok);
-    uc[i] = load_row<VEC>(u + id, ok);
-  }
-#pragma unroll
-  for (int i = 0; i < RY; ++i) {
-    const int y = y0 + i;
-    const bool ok = xok && y < ny;
-    const int id = ok ? y * nx + x0 : 0;
-    zz[i] = load_row<VEC>(z2 + id, ok);
-    ac[i] = load_row<VEC>(acc + id, ok);
-    if (COV) cv[i] = load_row<VEC>(cov + id, ok);
-    load_row_int<VEC>(A.cellmap + id, ok, cm[i]);
-  }
-  float zl[RY], zr[RY], pl[RY], pr[RY], ul[RY], ur[RY];
-#pragma unroll
-  for (int i = 0; i < RY; ++i) {
-    const int y = y0 + i;
-    const bool ok = xok && y < ny;
-    const int rb = ok ? y * nx : 0;
-    row_ends<VEC>(zc[i + 1], z1 + rb, x0, nx, ok, zl[i], zr[i]);
-    row_ends<VEC>(pc[i + 1], pw + rb, x0, nx, ok, pl[i], pr[i]);
-    row_ends<VEC>(uc[i + 1], u + rb, x0, nx, ok, ul[i], ur[i]);
+  for (int j = 0; j < PD; ++j) {
+    const int yw = y0 + 1 + j, ys = y0 + j;
+    qz[j] = load_row<VEC>(z1 + at(yw, wok(yw)), wok(yw));
+    qp[j] = load_row<VEC>(pw + at(yw, wok(yw)), wok(yw));
+    qu[j] = load_row<VEC>(u + at(yw, wok(yw)), wok(yw));
+    q2[j] = load_row<VEC>(z2 + at(ys, sok(ys)), sok(ys));
+    qa[j] = load_row<VEC>(acc + at(ys, sok(ys)), sok(ys));
+    if (COV) qc[j] = load_row<VEC>(cov + at(ys, sok(ys)), sok(ys));
   }
 
+#pragma unroll 1
+  for (int base = y0; base < yend; base += PD) {
+    Row<VEC> nz[PD], np[PD], nu[PD], n2r[PD], na[PD], nc[PD];
 #pragma unroll
-  for (int i = 0; i < RY; ++i) {
-    const int y = y0 + i;
-    if (!xok || y >= ny) continue;
-    Row<VEC> zo, ao, co;
-#pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-      const int x = x0 + e;
-      const bool interior = x > 0 && x < nx - 1 && y > 0 && y < ny - 1;
-      const int ri = interior ? cm[i][e] : -1;
-      float zval, lu, uval;
-      if (interior && ri < 0) {
-        float a, ab;
-        sponge_factors<SPONGE>(A, x, y, a, ab);
-        const float zle = e == 0 ? zl[i] : zc[i + 1].e[e - 1];
-        const float zre = e == VEC - 1 ? zr[i] : zc[i + 1].e[e + 1];
-        const float ple = e == 0 ? pl[i] : pc[i + 1].e[e - 1];
-        const float pre = e == VEC - 1 ? pr[i] : pc[i + 1].e[e + 1];
-        const float ule = e == 0 ? ul[i] : uc[i + 1].e[e - 1];
-        const float ure = e == VEC - 1 ? ur[i] : uc[i + 1].e[e + 1];
-        zval = (((pc[i].e[e] * zc[i].e[e] + pc[i + 2].e[e] * zc[i + 2].e[e]) + ple * zle) + pre * zre) +
-               (2.f * a - 4.f * pc[i + 1].e[e]) * zc[i + 1].e[e] - ab * zz[i].e[e];
-        lu = (((uc[i].e[e] + uc[i + 2].e[e]) + ule) + ure - 4.f * uc[i + 1].e[e]) * A.inv_h2;
-        uval = uc[i + 1].e[e];
-      } else {
-        adjoint_slow<SPONGE>(A, z1, z2, u, gs, x, y, ri, zval, lu, uval);
-      }
-      zo.e[e] = zval;
-      ao.e[e] = ac[i].e[e] + zval * lu;
-      if (COV) co.e[e] = cv[i].e[e] + uval * uval;
+    for (int j = 0; j < PD; ++j) {
+      const int yw = base + PD + 1 + j, ys = base + PD + j;
+      nz[j] = load_row<VEC>(z1 + at(yw, wok(yw)), wok(yw));
+      np[j] = load_row<VEC>(pw + at(yw, wok(yw)), wok(yw));
+      nu[j] = load_row<VEC>(u + at(yw, wok(yw)), wok(yw));
+      n2r[j] = load_row<VEC>(z2 + at(ys, sok(ys)), sok(ys));
+      na[j] = load_row<VEC>(acc + at(ys, sok(ys)), sok(ys));
+      if (COV) nc[j] = load_row<VEC>(cov + at(ys, sok(ys)), sok(ys));
     }
-    const int id = y * nx + x0;
-    store_row<VEC>(z0 + id, zo);
-    store_row<VEC>(acc + id, ao);
-    if (COV) store_row<VEC>(cov + id, co);
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      const int y = base + j;
+      if (y < yend) {  // wavefront-uniform
+        float zl, zr, pl, pr, ul, ur;
+        const int rb = xok ? y * nx : 0;
+        row_ends<VEC>(zc, z1 + rb, x0, nx, xok, zl, zr);
+        row_ends<VEC>(pc, pw + rb, x0, nx, xok, pl, pr);
+        row_ends<VEC>(uc, u + rb, x0, nx, xok, ul, ur);
+        if (xok && y > 0 && y < ny - 1) {
+          // receivers of this row (rare): one flag per row, then the row's cells
+          int skip = 0;
+          if (A.row_has_recv[y]) {
+            int cm[VEC];
+            load_row_int<VEC>(A.cellmap + y * nx + x0, true, cm);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) skip |= (cm[e] >= 0) << e;
+          }
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const int x = x0 + e;
+            if (x == 0 || x == nx - 1) skip |= 1 << e;
+          }
+          Row<VEC> zo, ao, co;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+            const int x = x0 + e;
+            float a, ab;
+            sponge_factors<SPONGE>(A, x, y, a, ab);
+            const float zle = e == 0 ? zl : zc.e[e - 1];
+            const float zre = e == VEC - 1 ? zr : zc.e[e + 1];
+            const float ple = e == 0 ? pl : pc.e[e - 1];
+            const float pre = e == VEC - 1 ? pr : pc.e[e + 1];
+            const float ule = e == 0 ? ul : uc.e[e - 1];
+            const float ure = e == VEC - 1 ? ur : uc.e[e + 1];
+            const float zval = (((pu.e[e] * zu.e[e] + qp[j].e[e] * qz[j].e[e]) + ple * zle) + pre * zre) +
+                               (2.f * a - 4.f * pc.e[e]) * zc.e[e] - ab * q2[j].e[e];
+            const float lu = (((uu.e[e] + qu[j].e[e]) + ule) + ure - 4.f * uc.e[e]) * A.inv_h2;
+            zo.e[e] = zval;
+            ao.e[e] = qa[j].e[e] + zval * lu;
+            if (COV) co.e[e] = qc[j].e[e] + uc.e[e] * uc.e[e];
+          }
+          const int id = y * nx + x0;
+          if (!skip) {
+            store_row<VEC>(z0 + id, zo);
+            store_row<VEC>(acc + id, ao);
+            if (COV) store_row<VEC>(cov + id, co);
+          } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              if (skip & (1 << e)) continue;
+              z0[id + e] = zo.e[e];
+              acc[id + e] = ao.e[e];
+              if (COV) cov[id + e] = co.e[e];
+            }
+          }
+        }
+      }
+      zu = zc; zc = qz[j];
+      pu = pc; pc = qp[j];
+      uu = uc; uc = qu[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      qz[j] = nz[j];
+      qp[j] = np[j];
+      qu[j] = nu[j];
+      q2[j] = n2r[j];
+      qa[j] = na[j];
+      if (COV) qc[j] = nc[j];
+    }
   }
+}
+
+// Edge, corner and receiver cells of one adjoint step for the vectorised
+// kernels (the cells adjoint_vec_kernel skips); inputs are the previous
+// levels only, so the order against adjoint_vec_kernel does not matter.
+template <bool SPONGE, bool COV>
+__global__ __launch_bounds__(256) void adjoint_ring_kernel(AdjArgs A, const int2 *__restrict__ cells, int ncells) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  const int2 c = cells[i];
+  const int gs = A.shot0 + blockIdx.y;
+  const long long so = (long long)blockIdx.y * A.sstride;
+  const long long n2 = (long long)A.nx * A.ny;
+  const int x = c.x % A.nx, y = c.x / A.nx;
+  float zval, lu, uval;
+  adjoint_slow<SPONGE>(A, A.z1 + so, A.z2 + so, A.u + so, gs, x, y, c.y, zval, lu, uval);
+  A.z0[so + c.x] = zval;
+  A.acc[gs * n2 + c.x] += zval * lu;
+  if (COV) A.cov[gs * n2 + c.x] += uval * uval;
 }
 
 // ------------------------------------------------------------- elementwise
@@ -711,15 +832,28 @@ inline dim3 vec_grid(int nx, int ny, int vec, int ry, int nshots, int *tiles_x, 
   return dim3(*tiles_per_shot * nshots, 1, 1);
 }
 
+template <bool SPONGE, int VEC, int PD>
+void fwd_vec_rows(FwdArgs<float> a, dim3 grid, dim3 block, int ry, hipStream_t st) {
+  switch (ry) {
+    case 4: forward_vec_kernel<SPONGE, VEC, 4, PD><<<grid, block, 0, st>>>(a); break;
+    case 8: forward_vec_kernel<SPONGE, VEC, 8, PD><<<grid, block, 0, st>>>(a); break;
+    case 32: forward_vec_kernel<SPONGE, VEC, 32, PD><<<grid, block, 0, st>>>(a); break;
+    default: forward_vec_kernel<SPONGE, VEC, 16, PD><<<grid, block, 0, st>>>(a); break;
+  }
+}
+
 template <bool SPONGE, int VEC>
-hipError_t fwd_vec_dispatch(FwdArgs<float> a, int nshots, int ry, hipStream_t st) {
+hipError_t fwd_vec_dispatch(FwdArgs<float> a, int nshots, int ry, int pd, const int2 *ring, int nring,
+                            hipStream_t st) {
   const dim3 block(kBlockX, kBlockY, 1);
   const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x, &a.tiles_per_shot);
-  switch (ry) {
-    case 2: forward_vec_kernel<SPONGE, VEC, 2><<<grid, block, 0, st>>>(a); break;
-    case 8: forward_vec_kernel<SPONGE, VEC, 8><<<grid, block, 0, st>>>(a); break;
-    default: forward_vec_kernel<SPONGE, VEC, 4><<<grid, block, 0, st>>>(a); break;
+  switch (pd) {
+    case 1: fwd_vec_rows<SPONGE, VEC, 1>(a, grid, block, ry, st); break;
+    case 4: fwd_vec_rows<SPONGE, VEC, 4>(a, grid, block, ry, st); break;
+    default: fwd_vec_rows<SPONGE, VEC, 2>(a, grid, block, ry, st); break;
   }
+  // edge cells and receiver record read this step's interior result: after
+  if (nring > 0) forward_ring_kernel<SPONGE><<<dim3((nring + 255) / 256, nshots, 1), 256, 0, st>>>(a, ring, nring);
   return hipGetLastError();
 }
 
@@ -734,14 +868,25 @@ hipError_t adj_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <bool SPONGE, bool COV, int VEC, int PD>
+void adj_vec_rows(AdjArgs a, dim3 grid, dim3 block, int ry, hipStream_t st) {
+  switch (ry) {
+    case 4: adjoint_vec_kernel<SPONGE, COV, VEC, 4, PD><<<grid, block, 0, st>>>(a); break;
+    case 8: adjoint_vec_kernel<SPONGE, COV, VEC, 8, PD><<<grid, block, 0, st>>>(a); break;
+    case 32: adjoint_vec_kernel<SPONGE, COV, VEC, 32, PD><<<grid, block, 0, st>>>(a); break;
+    default: adjoint_vec_kernel<SPONGE, COV, VEC, 16, PD><<<grid, block, 0, st>>>(a); break;
+  }
+}
+
 template <bool SPONGE, bool COV, int VEC>
-hipError_t adj_vec_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
+hipError_t adj_vec_dispatch(AdjArgs a, int nshots, int ry, int pd, const int2 *ring, int nring, hipStream_t st) {
   const dim3 block(kBlockX, kBlockY, 1);
   const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x, &a.tiles_per_shot);
-  switch (ry) {
-    case 1: adjoint_vec_kernel<SPONGE, COV, VEC, 1><<<grid, block, 0, st>>>(a); break;
-    case 4: adjoint_vec_kernel<SPONGE, COV, VEC, 4><<<grid, block, 0, st>>>(a); break;
-    default: adjoint_vec_kernel<SPONGE, COV, VEC, 2><<<grid, block, 0, st>>>(a); break;
+  if (nring > 0) adjoint_ring_kernel<SPONGE, COV><<<dim3((nring + 255) / 256, nshots, 1), 256, 0, st>>>(a, ring, nring);
+  switch (pd) {
+    case 1: adj_vec_rows<SPONGE, COV, VEC, 1>(a, grid, block, ry, st); break;
+    case 4: adj_vec_rows<SPONGE, COV, VEC, 4>(a, grid, block, ry, st); break;
+    default: adj_vec_rows<SPONGE, COV, VEC, 2>(a, grid, block, ry, st); break;
   }
   return hipGetLastError();
 }
@@ -762,10 +907,13 @@ hipError_t launch_forward_step<float>(const FwdArgs<float> &a, int nshots, const
   const int vec = sh.vec;
   if (vec == 2 || vec == 4) {
     if (a.nx % vec != 0) return hipErrorInvalidValue;
-    const int ry = sh.rows == 2 || sh.rows == 8 ? sh.rows : 4;
+    const int ry = (sh.rows == 4 || sh.rows == 8 || sh.rows == 32) ? sh.rows : 16;
     const bool sp = a.sponge_w > 0;
-    if (vec == 4) return sp ? fwd_vec_dispatch<true, 4>(a, nshots, ry, st) : fwd_vec_dispatch<false, 4>(a, nshots, ry, st);
-    return sp ? fwd_vec_dispatch<true, 2>(a, nshots, ry, st) : fwd_vec_dispatch<false, 2>(a, nshots, ry, st);
+    if (vec == 4)
+      return sp ? fwd_vec_dispatch<true, 4>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st)
+                : fwd_vec_dispatch<false, 4>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st);
+    return sp ? fwd_vec_dispatch<true, 2>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st)
+              : fwd_vec_dispatch<false, 2>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st);
   }
   int ry = sh.rows;
   if (ry != 4 && ry != 8 && ry != 16) ry = 8;
@@ -778,10 +926,13 @@ hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape 
   const int vec = sh.vec;
   if (vec == 2 || vec == 4) {
     if (a.nx % vec != 0) return hipErrorInvalidValue;
-    const int ry = sh.rows == 1 || sh.rows == 4 ? sh.rows : 2;
-#define TOMO_ADJ_VEC(V)                                                                          \
-  if (sp) return cov ? adj_vec_dispatch<true, true, V>(a, nshots, ry, st) : adj_vec_dispatch<true, false, V>(a, nshots, ry, st); \
-  return cov ? adj_vec_dispatch<false, true, V>(a, nshots, ry, st) : adj_vec_dispatch<false, false, V>(a, nshots, ry, st);
+    const int ry = (sh.rows == 4 || sh.rows == 8 || sh.rows == 32) ? sh.rows : 16;
+#define TOMO_ADJ_VEC(V)                                                                                   \
+  if (sp)                                                                                                 \
+    return cov ? adj_vec_dispatch<true, true, V>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st)          \
+               : adj_vec_dispatch<true, false, V>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st);        \
+  return cov ? adj_vec_dispatch<false, true, V>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st)           \
+             : adj_vec_dispatch<false, false, V>(a, nshots, ry, sh.depth, sh.ring, sh.nring, st);
     if (vec == 4) {
       TOMO_ADJ_VEC(4)
     }
