@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows,fwd_depth,adj_depth (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, copy 1 GiB device to device (a plain copy of known size, used to scale "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -187,6 +190,14 @@ def main():
                       f"{shots} shots, {n}x{n}, {steps} steps: {t_cpu:.2f} s",
         }
     e.close()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        print("[bench] copy calibration: 1 GiB device copy", file=sys.stderr, flush=True)
+        b.copy_(a)
+        torch.cuda.synchronize()
     print(json.dumps(out), flush=True)
 
 
