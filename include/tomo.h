@@ -220,6 +220,11 @@ int tomo_sides_to_traces(const tomo_ctx *ctx, const float *bottom, const float *
 
 /* -------------------------------------------------------------- gradient */
 
+/* Data misfit sum_s sum_k sum_r (meas - u)^2 of all shots at the current
+ * field: the forward half of tomo_gradient (sirt-mimo.cu:617-684 record and
+ * difference), without the adjoint; for step-length searches. */
+int tomo_misfit(tomo_ctx *ctx, double *misfit);
+
 /* For shots [g0, g0+ng): forward on the current field, residual
  * measured - simulated at the receivers, time-reversed adjoint with residual
  * injection, and the correlation G_g(i,j) = sum_k z(i,j,k) * Lu(i,j,k)
@@ -304,6 +309,72 @@ int tomo_timing_reset(tomo_ctx *ctx);
 int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth);
 /* out5 = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth} in effect */
 int tomo_get_shape(const tomo_ctx *ctx, int *out5);
+
+/* ------------------------------------------------------------------ learned
+ * Learned preconditioner.  No counterpart in the reference (which has no
+ * learned component): a 5-layer 3x3 convolutional network, channels
+ * 3 -> 32 -> 32 -> 32 -> 32 -> 1 with ReLU after layers 1-4, bfloat16
+ * weights and activations on the matrix cores with FP32 accumulation.  Its
+ * input at each pixel is (field / 0.1, plain SIRT step / 0.1, summed coverage
+ * / its maximum), the plain SIRT step being scale * mask * (sum_s G_s) /
+ * (1 + f) / S for the relaxation `scale` of the plain iteration; its output,
+ * times 0.1 and the imaging mask, is a field update.  Trained with Adam on
+ * samples taken from plain SIRT trajectories, target (truth - field) / 0.1
+ * over the mask.
+ *
+ * Parameters (FP32, 28929 values): per layer W[o][c][ky][kx] then b[o],
+ * y[p, o] = b[o] + sum W[o][c][ky][kx] x[p + (ky - 1, kx - 1), c], zero
+ * padding.  Image width must be a multiple of 32.
+ */
+typedef struct tomo_cnn tomo_cnn;
+
+int tomo_cnn_num_params(void);
+/* network on `device` (-1: current), initialised with seed 1 */
+int tomo_cnn_create(int device, tomo_cnn **out);
+void tomo_cnn_destroy(tomo_cnn *net);
+const char *tomo_cnn_last_error(const tomo_cnn *net);
+/* He-normal weights (output layer scaled by 0.1), zero biases; resets Adam */
+int tomo_cnn_init(tomo_cnn *net, uint64_t seed);
+int tomo_cnn_set_params(tomo_cnn *net, const float *params);  /* resets Adam */
+int tomo_cnn_get_params(tomo_cnn *net, float *params);
+/* y[B][H][W] from x[B][H][W][3] (host arrays) */
+int tomo_cnn_forward(tomo_cnn *net, int batch, int h, int w, const float *x, float *y);
+/* grad[P] = d(sum dy * y)/d(params) for y = net(x) (host arrays) */
+int tomo_cnn_backward(tomo_cnn *net, int batch, int h, int w, const float *x, const float *dy, float *grad);
+/* masked mean-square loss sum_{mask} (y - t)^2 / (2 B |mask|), mask4 =
+ * {x0, x1, y0, y1} half-open; grad may be NULL */
+int tomo_cnn_loss(tomo_cnn *net, int batch, int h, int w, const float *x, const float *target, const int *mask4,
+                  double *loss, float *grad);
+/* training samples from an engine context: its current field, the plain
+ * SIRT step with relaxation `scale` from the gradients of all shots
+ * (tomo_gradient) and the coverage, target from its truth; all samples share
+ * one grid size and mask */
+int tomo_cnn_add_sample(tomo_cnn *net, tomo_ctx *ctx, float scale);
+int tomo_cnn_num_samples(const tomo_cnn *net);
+int tomo_cnn_clear_samples(tomo_cnn *net);
+/* `steps` Adam steps (beta 0.9 / 0.999) on random batches; losses[steps] optional */
+int tomo_cnn_train(tomo_cnn *net, int steps, int batch, double lr, uint64_t seed, double *losses);
+/* f += mask * (alpha * 0.1 * net(input) + max(1 - alpha, 0) * plain step),
+ * the input from the context's state with the plain step of relaxation
+ * `scale` (alpha = 1: the learned step alone; alpha -> 0: plain SIRT);
+ * needs the gradients of all shots at the current field */
+int tomo_learned_update(tomo_ctx *ctx, tomo_cnn *net, float alpha, float scale);
+/* One learned iteration with the plain fallback: the gradient of all shots
+ * at the current field f (misfit m0); the plain SIRT step from f (uniform
+ * MEAN, or weighted when `weighted`, with relaxation `scale`) is kept as the
+ * fallback; the learned candidate of tomo_learned_update with factor a is
+ * formed and its data misfit m1 computed (forward only).  If m1 < m0 the
+ * candidate is the new field (*accepted = 1, a grows by 1.5x up to `alpha`),
+ * else the plain step is (*accepted = 0, a halves).  a starts at `alpha` on
+ * a new context.  *misfit = m0; *eps = relative error (%) of the new field
+ * when the context has a truth.  Cost: one gradient and one forward pass. */
+int tomo_learned_iteration(tomo_ctx *ctx, tomo_cnn *net, float alpha, float scale, int weighted, double *misfit,
+                           double *eps, int *accepted);
+/* forget a pending learned step (start of a new run) */
+int tomo_learned_reset(tomo_cnn *net);
+/* while on, each tomo_learned_iteration also adds a training sample (as
+ * tomo_cnn_add_sample) at the state its gradient was taken at */
+int tomo_cnn_set_sampling(tomo_cnn *net, int on);
 
 #ifdef __cplusplus
 }

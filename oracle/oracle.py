@@ -31,6 +31,11 @@ orc_update_weighted_*      weights sum to 1, permutation symmetry, degenerate
 orc_update_group_*         closed form
 orc_error_*                closed form
 orc_schedule               closed forms of both schedules
+orc_cnn_forward            a hand-computed 1-pixel / delta-input network (closed form
+                           of the zero-padded cross-correlation); per-layer shapes
+orc_cnn_backward /         finite differences of orc_cnn_loss in FP64 along random
+orc_cnn_loss               directions (a mirrored-tap mistake fails it); FP64 vs
+                           BF16-rounding modes agree to BF16 precision
 =========================  ==========================================================
 """
 from __future__ import annotations
@@ -96,6 +101,13 @@ def lib():
             fn.restype = ctypes.c_double
             fn = getattr(L, "orc_pulse_" + suf)
             fn.argtypes = [vp, p]
+        L.orc_cnn_num_params.restype = ctypes.c_int
+        L.orc_cnn_forward.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, ctypes.c_int]
+        L.orc_cnn_backward.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f64p,
+                                       ctypes.c_int]
+        L.orc_cnn_loss.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _i32p, vp,
+                                   ctypes.c_int]
+        L.orc_cnn_loss.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -246,3 +258,37 @@ def laplacian(cfg, u):
 
 def schedule(kind, k, params):
     return float(lib().orc_schedule(int(kind), int(k), np.ascontiguousarray(params, np.float64)))
+
+
+# ------------------------------------------------------- learned preconditioner
+
+def cnn_num_params() -> int:
+    return int(lib().orc_cnn_num_params())
+
+
+def cnn_forward(params, x, mode=0):
+    """x[B][H][W][3] float32 -> y[B][H][W] float32 (mode 1: the GPU's BF16 rounding points)."""
+    B, H, W, _ = x.shape
+    y = np.zeros((B, H, W), np.float32)
+    lib().orc_cnn_forward(np.ascontiguousarray(params, np.float32), B, H, W, np.ascontiguousarray(x, np.float32),
+                          y, int(mode))
+    return y
+
+
+def cnn_backward(params, x, dy, mode=0):
+    """Gradient of sum(dy * y) with respect to the parameters (float64)."""
+    B, H, W, _ = x.shape
+    g = np.zeros(cnn_num_params(), np.float64)
+    lib().orc_cnn_backward(np.ascontiguousarray(params, np.float32), B, H, W, np.ascontiguousarray(x, np.float32),
+                           np.ascontiguousarray(dy, np.float32), g, int(mode))
+    return g
+
+
+def cnn_loss(params, x, target, mask4, mode=0, grad=True):
+    """Masked mean-square loss and (optionally) its parameter gradient."""
+    B, H, W, _ = x.shape
+    g = np.zeros(cnn_num_params(), np.float64) if grad else None
+    m = np.ascontiguousarray(mask4, np.int32)
+    loss = lib().orc_cnn_loss(np.ascontiguousarray(params, np.float32), B, H, W, np.ascontiguousarray(x, np.float32),
+                              np.ascontiguousarray(target, np.float32), m, _ptr(g), int(mode))
+    return (loss, g) if grad else loss

@@ -347,11 +347,11 @@ except AssertionError:
 @pytest.mark.parametrize("mutation,check", [(1, "kat"), (2, "kat"), (3, "kat"), (4, "dot")])
 def test_planted_mistakes_are_caught(mutation, check):
     """The same KAT / dot-product test must fail on an oracle with a planted mistake."""
-    src = os.path.join(orc.HERE, "tomo_oracle.cpp")
+    src = [os.path.join(orc.HERE, f) for f in ("tomo_oracle.cpp", "cnn_oracle.cpp")]
     with tempfile.TemporaryDirectory() as td:
         so = os.path.join(td, "liboracle_mut.so")
         subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
-                        f"-DORC_MUTATION={mutation}", "-o", so, src], check=True)
+                        f"-DORC_MUTATION={mutation}", "-o", so, *src], check=True)
         script = _MUTANT_SCRIPT.format(tests=HERE, oracle=orc.HERE)
         env = dict(os.environ, TOMO_ORACLE_LIB=so)
         out = subprocess.run([sys.executable, "-c", script, check], env=env, check=True,

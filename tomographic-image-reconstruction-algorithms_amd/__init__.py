@@ -104,6 +104,7 @@ SIGNATURES = {
     "tomo_traces_to_sides": (_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
     "tomo_sides_to_traces": (_int, [_vp, _f32p, _f32p, _f32p, _f32p, _f32p]),
     "tomo_gradient": (_int, [_vp, _int, _int, _vp]),
+    "tomo_misfit": (_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "tomo_get_gradient": (_int, [_vp, _int, _f32p]),
     "tomo_enable_coverage": (_int, [_vp, _int]),
     "tomo_get_coverage": (_int, [_vp, _f32p]),
@@ -120,6 +121,25 @@ SIGNATURES = {
     "tomo_timing_reset": (_int, [_vp]),
     "tomo_set_shape": (_int, [_vp, _int, _int, _int, _int, _int]),
     "tomo_get_shape": (_int, [_vp, _i32p]),
+    # learned preconditioner
+    "tomo_cnn_num_params": (_int, []),
+    "tomo_cnn_create": (_int, [_int, ctypes.POINTER(_vp)]),
+    "tomo_cnn_destroy": (None, [_vp]),
+    "tomo_cnn_last_error": (ctypes.c_char_p, [_vp]),
+    "tomo_cnn_init": (_int, [_vp, ctypes.c_uint64]),
+    "tomo_cnn_set_params": (_int, [_vp, _f32p]),
+    "tomo_cnn_get_params": (_int, [_vp, _f32p]),
+    "tomo_cnn_forward": (_int, [_vp, _int, _int, _int, _f32p, _f32p]),
+    "tomo_cnn_backward": (_int, [_vp, _int, _int, _int, _f32p, _f32p, _f32p]),
+    "tomo_cnn_loss": (_int, [_vp, _int, _int, _int, _f32p, _f32p, _i32p, ctypes.POINTER(ctypes.c_double), _vp]),
+    "tomo_cnn_add_sample": (_int, [_vp, _vp, ctypes.c_float]),
+    "tomo_cnn_num_samples": (_int, [_vp]),
+    "tomo_cnn_clear_samples": (_int, [_vp]),
+    "tomo_cnn_train": (_int, [_vp, _int, _int, ctypes.c_double, ctypes.c_uint64, _vp]),
+    "tomo_learned_update": (_int, [_vp, _vp, ctypes.c_float, ctypes.c_float]),
+    "tomo_learned_iteration": (_int, [_vp, _vp, ctypes.c_float, ctypes.c_float, _int, _vp, _vp, _vp]),
+    "tomo_learned_reset": (_int, [_vp]),
+    "tomo_cnn_set_sampling": (_int, [_vp, _int]),
 }
 
 
@@ -138,13 +158,16 @@ def lib():
     return _lib
 
 
-def _check(status, ctx=None):
+def _check(status, ctx=None, net=None):
     if status < 0:
         msg = _STATUS.get(status, str(status))
+        detail = None
         if ctx is not None:
             detail = lib().tomo_last_error(ctx)
-            if detail:
-                msg += ": " + detail.decode(errors="replace")
+        elif net is not None:
+            detail = lib().tomo_cnn_last_error(net)
+        if detail:
+            msg += ": " + detail.decode(errors="replace")
         raise TomoError(msg)
     return status
 
@@ -330,6 +353,12 @@ class Engine:
         self._c(lib().tomo_gradient(self._ctx, int(g0), int(ng), ctypes.cast(ctypes.byref(mis), _vp)))
         return mis.value
 
+    def misfit(self) -> float:
+        """Data misfit of all shots at the current field (forward only)."""
+        m = ctypes.c_double(0.0)
+        self._c(lib().tomo_misfit(self._ctx, ctypes.byref(m)))
+        return m.value
+
     def get_gradient(self, g):
         out = np.zeros(self.n2, np.float32)
         self._c(lib().tomo_get_gradient(self._ctx, int(g), out))
@@ -393,3 +422,112 @@ class Engine:
         out = np.zeros(5, np.int32)
         self._c(lib().tomo_get_shape(self._ctx, out))
         return tuple(int(v) for v in out)
+
+
+# ------------------------------------------------------- learned preconditioner
+
+class Preconditioner:
+    """The learned preconditioner network (one HIP stream on one GPU).
+
+    ``x`` arrays are float32 [B][H][W][3] (field / 0.1, plain SIRT step / 0.1,
+    coverage / max); outputs float32 [B][H][W]; parameters float32 in the
+    layout W[o][c][ky][kx] then b[o], layer by layer (28929 values).
+    """
+
+    def __init__(self, device: int = -1, seed: int | None = None):
+        self._net = _vp()
+        _check(lib().tomo_cnn_create(int(device), ctypes.byref(self._net)))
+        self.num_params = int(lib().tomo_cnn_num_params())
+        if seed is not None:
+            self.init(seed)
+
+    def close(self):
+        if self._net:
+            lib().tomo_cnn_destroy(self._net)
+            self._net = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, status):
+        return _check(status, net=self._net)
+
+    def init(self, seed: int):
+        self._c(lib().tomo_cnn_init(self._net, int(seed)))
+
+    def set_params(self, p):
+        p = _f32(p)
+        if p.size != self.num_params:
+            raise TomoError(f"expected {self.num_params} parameters, got {p.size}")
+        self._c(lib().tomo_cnn_set_params(self._net, p))
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        self._c(lib().tomo_cnn_get_params(self._net, p))
+        return p
+
+    def forward(self, x):
+        x = _f32(x)
+        B, H, W, _ = x.shape
+        y = np.zeros((B, H, W), np.float32)
+        self._c(lib().tomo_cnn_forward(self._net, B, H, W, x, y))
+        return y
+
+    def backward(self, x, dy):
+        x = _f32(x)
+        B, H, W, _ = x.shape
+        g = np.zeros(self.num_params, np.float32)
+        self._c(lib().tomo_cnn_backward(self._net, B, H, W, x, _f32(dy), g))
+        return g
+
+    def loss(self, x, target, mask4, grad=True):
+        x = _f32(x)
+        B, H, W, _ = x.shape
+        g = np.zeros(self.num_params, np.float32) if grad else None
+        L = ctypes.c_double(0.0)
+        self._c(lib().tomo_cnn_loss(self._net, B, H, W, x, _f32(target), _i32(mask4), ctypes.byref(L), _ptr(g)))
+        return (L.value, g) if grad else L.value
+
+    def add_sample(self, engine: "Engine", scale: float):
+        self._c(lib().tomo_cnn_add_sample(self._net, engine._ctx, float(scale)))
+
+    def num_samples(self) -> int:
+        return int(lib().tomo_cnn_num_samples(self._net))
+
+    def clear_samples(self):
+        self._c(lib().tomo_cnn_clear_samples(self._net))
+
+    def train(self, steps: int, batch: int, lr: float, seed: int = 0):
+        losses = np.zeros(max(int(steps), 1), np.float64)
+        self._c(lib().tomo_cnn_train(self._net, int(steps), int(batch), float(lr), int(seed), _ptr(losses)))
+        return losses[: int(steps)]
+
+    def update(self, engine: "Engine", alpha: float, scale: float):
+        """f += mask * (alpha * 0.1 * net(input) + max(1 - alpha, 0) * plain step with `scale`)."""
+        engine._c(lib().tomo_learned_update(engine._ctx, self._net, float(alpha), float(scale)))
+
+    def iteration(self, engine: "Engine", alpha: float, scale: float, weighted: bool = True):
+        """One learned iteration with the plain-SIRT fallback; returns (misfit, eps, accepted)."""
+        mis = ctypes.c_double(0.0)
+        eps = ctypes.c_double(0.0)
+        acc = ctypes.c_int(0)
+        engine._c(lib().tomo_learned_iteration(engine._ctx, self._net, float(alpha), float(scale), int(bool(weighted)),
+                                               ctypes.cast(ctypes.byref(mis), _vp), ctypes.cast(ctypes.byref(eps), _vp),
+                                               ctypes.cast(ctypes.byref(acc), _vp)))
+        return mis.value, eps.value, bool(acc.value)
+
+    def reset(self):
+        self._c(lib().tomo_learned_reset(self._net))
+
+    def set_sampling(self, on: bool):
+        """While on, learned iterations add a training sample per visited state."""
+        self._c(lib().tomo_cnn_set_sampling(self._net, int(bool(on))))

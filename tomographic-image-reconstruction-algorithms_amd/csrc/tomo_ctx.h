@@ -1,0 +1,109 @@
+// tomo_ctx.h -- the engine context and the helpers that the engine and the
+// learned-preconditioner sources share (internal; the public ABI is
+// include/tomo.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "tomo.h"
+#include "tomo_kernels.h"
+
+struct TimedSpan {
+  hipEvent_t a, b;
+  int kind;          // 0 forward, 1 adjoint
+  double launches;
+  double cells;
+};
+
+struct tomo_ctx {
+  tomo_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nx = 0, ny = 0, nt = 0;
+  long long n2 = 0;
+  // geometry (host)
+  std::vector<int> tx, ty;  // transducers, 0-based
+  std::vector<int4> bbox;   // per shot
+  std::vector<int> rcell, ranchor;
+  int nshots = 0, nr = 0;
+  // persistent device buffers
+  float *d_f = nullptr, *d_fo = nullptr, *d_pw = nullptr;
+  double *d_pw64 = nullptr;
+  bool pw64_valid = false;
+  int *d_cellmap = nullptr, *d_anchor = nullptr;
+  int2 *d_ring = nullptr;                 // edge cells + receiver cells for the vectorised kernels
+  int nring = 0;
+  unsigned char *d_rowflag = nullptr;     // per row: holds a receiver
+  int4 *d_bbox = nullptr;
+  float *d_spa = nullptr, *d_spab = nullptr;
+  double *d_spa64 = nullptr, *d_spab64 = nullptr;
+  float *d_meas = nullptr, *d_rr = nullptr, *d_grad = nullptr, *d_cov = nullptr;
+  double *d_misfit = nullptr;
+  double *d_scratch = nullptr;
+  size_t scratch_n = 0;
+  // wavefield pool
+  float *d_ckpt = nullptr, *d_seg = nullptr, *d_z = nullptr;
+  size_t ckpt_n = 0, seg_n = 0, z_n = 0;
+  double *d_ring64 = nullptr;
+  size_t ring64_n = 0;
+  // state
+  bool have_truth = false, have_meas = false, coverage = false;
+  std::vector<char> grad_valid;
+  std::vector<float> amp32;   // pulse per step, float arithmetic
+  std::vector<double> amp64;  // pulse per step, double arithmetic
+  tomo::StencilShape shape_req{};  // requested fields (0 = automatic)
+  // timing
+  std::vector<hipEvent_t> ev_free;
+  std::vector<TimedSpan> spans;
+  double t_ms[2] = {0, 0}, t_launch[2] = {0, 0}, t_cells[2] = {0, 0};
+};
+
+namespace tomo_internal {
+
+inline int fail(tomo_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail((ctx), TOMO_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+inline int dalloc(tomo_ctx *c, T **p, size_t n) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) return TOMO_OK;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, TOMO_ERR_ALLOC, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  return TOMO_OK;
+}
+
+template <typename T>
+inline void dfree(T *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+int check_ctx(tomo_ctx *c);
+int need_shots(tomo_ctx *c);
+int need_all_gradients(tomo_ctx *c);
+int upload_coefficients(tomo_ctx *c);  // per-cell coefficients from the current field
+int ensure_scratch(tomo_ctx *c, size_t n);
+inline tomo::Mask ctx_mask(const tomo_ctx *c) {
+  return tomo::Mask{c->cfg.mask_x0, c->cfg.mask_x1, c->cfg.mask_y0, c->cfg.mask_y1};
+}
+
+}  // namespace tomo_internal

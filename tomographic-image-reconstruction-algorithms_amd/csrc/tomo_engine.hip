@@ -22,94 +22,12 @@
 #include "tomo.h"
 #include "tomo_host.h"
 #include "tomo_kernels.h"
+#include "tomo_ctx.h"
 
 using namespace tomo;
-
-struct TimedSpan {
-  hipEvent_t a, b;
-  int kind;          // 0 forward, 1 adjoint
-  double launches;
-  double cells;
-};
-
-struct tomo_ctx {
-  tomo_config cfg{};
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  int nx = 0, ny = 0, nt = 0;
-  long long n2 = 0;
-  // geometry (host)
-  std::vector<int> tx, ty;  // transducers, 0-based
-  std::vector<int4> bbox;   // per shot
-  std::vector<int> rcell, ranchor;
-  int nshots = 0, nr = 0;
-  // persistent device buffers
-  float *d_f = nullptr, *d_fo = nullptr, *d_pw = nullptr;
-  double *d_pw64 = nullptr;
-  bool pw64_valid = false;
-  int *d_cellmap = nullptr, *d_anchor = nullptr;
-  int2 *d_ring = nullptr;                 // edge cells + receiver cells for the vectorised kernels
-  int nring = 0;
-  unsigned char *d_rowflag = nullptr;     // per row: holds a receiver
-  int4 *d_bbox = nullptr;
-  float *d_spa = nullptr, *d_spab = nullptr;
-  double *d_spa64 = nullptr, *d_spab64 = nullptr;
-  float *d_meas = nullptr, *d_rr = nullptr, *d_grad = nullptr, *d_cov = nullptr;
-  double *d_misfit = nullptr;
-  double *d_scratch = nullptr;
-  size_t scratch_n = 0;
-  // wavefield pool
-  float *d_ckpt = nullptr, *d_seg = nullptr, *d_z = nullptr;
-  size_t ckpt_n = 0, seg_n = 0, z_n = 0;
-  double *d_ring64 = nullptr;
-  size_t ring64_n = 0;
-  // state
-  bool have_truth = false, have_meas = false, coverage = false;
-  std::vector<char> grad_valid;
-  std::vector<float> amp32;   // pulse per step, float arithmetic
-  std::vector<double> amp64;  // pulse per step, double arithmetic
-  StencilShape shape_req{};  // requested fields (0 = automatic)
-  // timing
-  std::vector<hipEvent_t> ev_free;
-  std::vector<TimedSpan> spans;
-  double t_ms[2] = {0, 0}, t_launch[2] = {0, 0}, t_cells[2] = {0, 0};
-};
+using namespace tomo_internal;
 
 namespace {
-
-int fail(tomo_ctx *c, int code, const std::string &msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, call)                                                                     \
-  do {                                                                                        \
-    hipError_t e_ = (call);                                                                   \
-    if (e_ != hipSuccess) return fail((ctx), TOMO_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-int dalloc(tomo_ctx *c, T **p, size_t n) {
-  if (*p) {
-    hipFree(*p);
-    *p = nullptr;
-  }
-  if (n == 0) return TOMO_OK;
-  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-  if (e != hipSuccess) {
-    *p = nullptr;
-    (void)hipGetLastError();
-    return fail(c, TOMO_ERR_ALLOC, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
-  }
-  return TOMO_OK;
-}
-
-template <typename T>
-void dfree(T *&p) {
-  if (p) hipFree(p);
-  p = nullptr;
-}
 
 int full_history(const tomo_ctx *c) {
   const int C = c->cfg.checkpoint_every;
@@ -128,7 +46,7 @@ StencilShape stencil_shape(const tomo_ctx *c) {
   s.vec = c->shape_req.vec;
   // 4-wide lanes when a 256-column tile is filled (or nearly), else 2-wide
   if (s.vec == 0) s.vec = (c->nx % 4 == 0 && (c->nx % 256 == 0 || c->nx > 256)) ? 4 : (c->nx % 2 == 0 ? 2 : 1);
-  if (c->nx % s.vec != 0) s.vec = 1;
+  if (c->nx % s.vec != 0 || c->nx < 4 * s.vec) s.vec = 1;
   // vectorised kernels stream a strip of rows per lane; taller strips cut
   // the halo re-read (2/rows) but leave fewer strips to fill the GPU.
   // Measured at 512^2 x 128 shots: forward 32 rows with 2 rows of loads in
@@ -194,6 +112,10 @@ void spans_collect(tomo_ctx *c) {
   c->spans.clear();
 }
 
+}  // namespace
+
+namespace tomo_internal {
+
 int upload_coefficients(tomo_ctx *c) {
   const double c0dt_h = c->cfg.c0 * c->cfg.dt / c->cfg.h;
   HIPCHK(c, launch_coefficients<float>(c->d_f, c->d_pw, c->nx, c->ny, c0dt_h, c->cfg.sponge_width, c->d_spa,
@@ -201,6 +123,11 @@ int upload_coefficients(tomo_ctx *c) {
   c->pw64_valid = false;
   return TOMO_OK;
 }
+
+}  // namespace tomo_internal
+
+namespace {
+
 
 int ensure_pw64(tomo_ctx *c) {
   if (c->pw64_valid) return TOMO_OK;
@@ -248,13 +175,11 @@ int upload_receivers(tomo_ctx *c) {
   std::vector<int> map((size_t)c->n2, -1);
   for (size_t r = 0; r < c->rcell.size(); ++r) map[c->rcell[r]] = (int)r;
   HIPCHK(c, hipMemcpyAsync(c->d_cellmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, c->stream));
-  // cells of the edge/receiver pass: the perimeter, then the receivers
+  // cells of the edge/receiver pass of the vectorised kernels: the top and
+  // bottom rows (the side edges of the other rows are done in the main
+  // kernels), then the receivers
   std::vector<int2> ring;
   for (int x = 0; x < c->nx; ++x) ring.push_back(make_int2(x, -1));
-  for (int y = 1; y < c->ny - 1; ++y) {
-    ring.push_back(make_int2(y * c->nx, -1));
-    ring.push_back(make_int2(y * c->nx + c->nx - 1, -1));
-  }
   for (int x = 0; x < c->nx; ++x) ring.push_back(make_int2((c->ny - 1) * c->nx + x, -1));
   std::vector<unsigned char> rowflag(c->ny, 0);
   for (size_t r = 0; r < c->rcell.size(); ++r) {
@@ -293,6 +218,10 @@ int alloc_shot_buffers(tomo_ctx *c) {
   return TOMO_OK;
 }
 
+}  // namespace
+
+namespace tomo_internal {
+
 int ensure_scratch(tomo_ctx *c, size_t n) {
   if (c->scratch_n >= n) return TOMO_OK;
   int rc = dalloc(c, &c->d_scratch, n);
@@ -300,6 +229,11 @@ int ensure_scratch(tomo_ctx *c, size_t n) {
   c->scratch_n = n;
   return TOMO_OK;
 }
+
+}  // namespace tomo_internal
+
+namespace {
+
 
 // wavefield storage for passes of P shots
 int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
@@ -404,10 +338,11 @@ int forward_step(tomo_ctx *c, FwdArgs<T> a, const T *um, const T *u0, T *up, int
   return TOMO_OK;
 }
 
-// traces of shots [g0, g0+ng) into d_rr (as u values), float or double solver
+// traces of shots [g0, g0+ng) into d_rr (as u values, or the residuals
+// meas - u when meas is given), float or double solver
 template <typename T>
 int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const T *pw, const T *spa,
-                   const T *spab) {
+                   const T *spab, const float *meas = nullptr) {
   const int P = std::min(pass_shots(c), ng);
   const size_t tr = (size_t)c->nt * c->nr;
   for (int p0 = g0; p0 < g0 + ng; p0 += P) {
@@ -425,7 +360,7 @@ int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const 
     TimedSpan s;
     span_begin(c, s, 0);
     for (int k = 1; k < c->nt - 1; ++k) {
-      int rc = forward_step<T>(c, a, L(k - 1), L(k), L(k + 1), k, np, true, nullptr, c->d_rr);
+      int rc = forward_step<T>(c, a, L(k - 1), L(k), L(k + 1), k, np, true, meas, c->d_rr);
       if (rc) return rc;
     }
     s.launches = c->nt - 2;
@@ -531,6 +466,10 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   return TOMO_OK;
 }
 
+}  // namespace
+
+namespace tomo_internal {
+
 int check_ctx(tomo_ctx *c) {
   if (!c) return TOMO_ERR_ARG;
   hipError_t e = hipSetDevice(c->device);
@@ -538,15 +477,38 @@ int check_ctx(tomo_ctx *c) {
   return TOMO_OK;
 }
 
+}  // namespace tomo_internal
+
+namespace {
+
+
+}  // namespace
+
+namespace tomo_internal {
+
 int need_shots(tomo_ctx *c) {
   if (c->nshots <= 0) return fail(c, TOMO_ERR_STATE, "no source groups: call tomo_set_groups first");
   if (c->nr <= 0) return fail(c, TOMO_ERR_STATE, "no receivers");
   return TOMO_OK;
 }
 
-Mask ctx_mask(const tomo_ctx *c) { return Mask{c->cfg.mask_x0, c->cfg.mask_x1, c->cfg.mask_y0, c->cfg.mask_y1}; }
+}  // namespace tomo_internal
+
+namespace {
+
+
 
 }  // namespace
+
+namespace tomo_internal {
+
+int need_all_gradients(tomo_ctx *c) {
+  for (int g = 0; g < c->nshots; ++g)
+    if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradients of all shots are required");
+  return TOMO_OK;
+}
+
+}  // namespace tomo_internal
 
 // ====================================================================== ABI
 extern "C" {
@@ -789,6 +751,30 @@ int tomo_forward(tomo_ctx *c, int g0, int ng, int fp64, float *traces) {
   return TOMO_OK;
 }
 
+int tomo_misfit(tomo_ctx *c, double *misfit) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  if (!misfit) return fail(c, TOMO_ERR_ARG, "null output");
+  if (!c->have_meas) return fail(c, TOMO_ERR_STATE, "no measured traces: call tomo_set_measured first");
+  const int P = pass_shots(c);
+  const size_t lvl = (size_t)P * c->n2;
+  if ((rc = ensure_wavefields(c, P, false))) return rc;
+  if ((rc = forward_traces<float>(c, 0, c->nshots, c->d_seg, lvl, c->d_pw, c->d_spa, c->d_spab, c->d_meas)))
+    return rc;
+  std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // d_rr now holds these residuals
+  const size_t tr = (size_t)c->nt * c->nr;
+  HIPCHK(c, launch_misfit(c->d_rr, (long long)tr, c->nshots, c->d_misfit, c->stream));
+  std::vector<double> mis(c->nshots);
+  HIPCHK(c, hipMemcpyAsync(mis.data(), c->d_misfit, sizeof(double) * c->nshots, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  spans_collect(c);
+  double sum = 0.0;
+  for (double v : mis) sum += v;
+  *misfit = sum;
+  return TOMO_OK;
+}
+
 int tomo_set_measured(tomo_ctx *c, const float *traces) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -864,11 +850,6 @@ int tomo_get_coverage(tomo_ctx *c, float *out) {
   return TOMO_OK;
 }
 
-static int need_all_gradients(tomo_ctx *c) {
-  for (int g = 0; g < c->nshots; ++g)
-    if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradients of all shots are required");
-  return TOMO_OK;
-}
 
 int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
   int rc = check_ctx(c);
@@ -991,7 +972,8 @@ int tomo_set_shape(tomo_ctx *c, int vec, int fwd_rows, int adj_rows, int fwd_dep
   if (!c) return TOMO_ERR_ARG;
   if (!shape_valid(vec, fwd_rows, adj_rows) || !depth_valid(fwd_depth) || !depth_valid(adj_depth))
     return fail(c, TOMO_ERR_ARG, "unsupported stencil shape");
-  if (vec > 1 && c->nx % vec != 0) return fail(c, TOMO_ERR_ARG, "nx is not a multiple of the vector width");
+  if (vec > 1 && (c->nx % vec != 0 || c->nx < 4 * vec))
+    return fail(c, TOMO_ERR_ARG, "nx is not a multiple of the vector width (or below 4 widths)");
   c->shape_req = StencilShape{};
   c->shape_req.vec = vec;
   c->shape_req.fwd_rows = fwd_rows;

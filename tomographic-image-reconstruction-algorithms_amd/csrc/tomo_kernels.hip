@@ -25,6 +25,14 @@ __device__ __forceinline__ int edge_dist(int x, int y, int nx, int ny) {
 
 // One-way edge rule of propagation (sirt-mimo.cu:566-587): corners use the
 // diagonal neighbour.
+// u(k+1) at an edge cell from u(k) at the cell (u), one (ua) and two (ub)
+// cells inward, and u(k-1) at the cell (m) and one cell inward (ma)
+template <typename T>
+__device__ __forceinline__ T mur_edge(T r, T u, T ua, T ub, T m, T ma) {
+  return (T(2) - T(2) * r - r * r) * u + T(2) * r * (T(1) + r) * ua - r * r * ub + (T(2) * r - T(1)) * m -
+         T(2) * r * ma;
+}
+
 template <typename T>
 __device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T *__restrict__ um,
                           int x, int y) {
@@ -37,8 +45,7 @@ __device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T
   const int ia = left ? x + 1 : right ? x - 1 : x;
   const int ib = left ? x + 2 : right ? x - 2 : x;
   const int ida = ja * nx + ia, idb = jb * nx + ib;
-  return (T(2) - T(2) * r - r * r) * u0[id] + T(2) * r * (T(1) + r) * u0[ida] - r * r * u0[idb] +
-         (T(2) * r - T(1)) * um[id] - T(2) * r * um[ida];
+  return mur_edge<T>(r, u0[id], u0[ida], u0[idb], um[id], um[ida]);
 }
 
 // Leapfrog step k -> k+1 for every cell of one tile of one shot
@@ -356,21 +363,10 @@ __device__ __forceinline__ void row_ends(const Row<VEC> &r, const float *__restr
 // window (rows above / at / below in registers) and issues the loads of the
 // next row before computing the current one, so a wavefront streams every
 // input row once (halo overhead (RY + 2) / RY) with few live registers.
-// Interior cells only: the edge cells and the receiver record are done by
-// forward_ring_kernel right after, so the hot loop has no cell map and no
-// slow path.
-template <int VEC>
-__device__ __forceinline__ void store_interior(float *__restrict__ p, const Row<VEC> &r, int x0, int nx) {
-  if (x0 > 0 && x0 + VEC < nx) {
-    store_row<VEC>(p, r);
-  } else {
-#pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-      const int x = x0 + e;
-      if (x > 0 && x < nx - 1) p[e] = r.e[e];
-    }
-  }
-}
+// Rows 1..ny-2 including their two side-edge cells (whose one-way rule needs
+// only cells of the same row); the top and bottom rows and the receiver
+// record are done by forward_ring_kernel right after, so the hot loop has no
+// cell map.  Needs nx >= 4 VEC.
 
 template <bool SPONGE, int VEC, int RY, int PD>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<float> A) {
@@ -449,7 +445,19 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<f
             if (src_row && x >= bb.x && x <= bb.y) val += pe * A.amp_h2;
             out.e[e] = val;
           }
-          store_interior<VEC>(up + y * nx + x0, out, x0, nx);
+          // side edges of this row: every operand is in this lane's registers
+          if (x0 == 0) {
+            const float r = A.c0dt_h * sqrtf(1.f + A.f[y * nx]);
+            out.e[0] = mur_edge<float>(r, cc.e[0], cc.e[1], VEC == 2 ? rgt : cc.e[2 % VEC], qm[j].e[0],
+                                       qm[j].e[1]);
+          }
+          if (x0 + VEC == nx) {
+            constexpr int E = VEC - 1;
+            const float r = A.c0dt_h * sqrtf(1.f + A.f[y * nx + nx - 1]);
+            out.e[E] = mur_edge<float>(r, cc.e[E], cc.e[E - 1], VEC == 2 ? lft : cc.e[(E + VEC - 2) % VEC],
+                                       qm[j].e[E], qm[j].e[E - 1]);
+          }
+          store_row<VEC>(up + y * nx + x0, out);
         }
       }
       cu = cc;
@@ -491,9 +499,10 @@ __global__ __launch_bounds__(256) void forward_ring_kernel(FwdArgs<float> A, con
 
 // Adjoint step, vectorised, same sliding-window structure: z(k+1), the
 // coefficients and u(k) stream through a three-row window; z(k+2) and the
-// accumulators are read for the current row only.  Interior cells that are
-// not receivers only (rows holding receivers are flagged per row; their
-// receiver cells are skipped): adjoint_ring_kernel does the rest.
+// accumulators are read for the current row only.  Rows 1..ny-2 including
+// their side-edge cells, receivers excepted (rows holding receivers are
+// flagged per row; their receiver cells are skipped): adjoint_ring_kernel
+// does the receivers and the top and bottom rows.
 template <bool SPONGE, bool COV, int VEC, int RY, int PD>
 __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A) {
   const int nx = A.nx, ny = A.ny;
@@ -570,11 +579,6 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A
 #pragma unroll
             for (int e = 0; e < VEC; ++e) skip |= (cm[e] >= 0) << e;
           }
-#pragma unroll
-          for (int e = 0; e < VEC; ++e) {
-            const int x = x0 + e;
-            if (x == 0 || x == nx - 1) skip |= 1 << e;
-          }
           Row<VEC> zo, ao, co;
 #pragma unroll
           for (int e = 0; e < VEC; ++e) {
@@ -593,6 +597,24 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A
             zo.e[e] = zval;
             ao.e[e] = qa[j].e[e] + zval * lu;
             if (COV) co.e[e] = qc[j].e[e] + uc.e[e] * uc.e[e];
+          }
+          // side edges (backpropagation2): z = the inward cell's value; Lu with
+          // clamped indices, the corner cells of rows 0 / ny-1 as the mean of
+          // their edge neighbours (laplace, laplace_corners)
+          if (x0 == 0) {
+            const float upv = y == 1 ? 0.5f * (uc.e[0] + uu.e[1]) : uu.e[0];
+            const float dnv = y == ny - 2 ? 0.5f * (uc.e[0] + qu[j].e[1]) : qu[j].e[0];
+            const float lu = (((upv + dnv) + uc.e[0]) + uc.e[1] - 4.f * uc.e[0]) * A.inv_h2;
+            zo.e[0] = zo.e[1];
+            ao.e[0] = qa[j].e[0] + zo.e[0] * lu;
+          }
+          if (x0 + VEC == nx) {
+            constexpr int E = VEC - 1;
+            const float upv = y == 1 ? 0.5f * (uc.e[E] + uu.e[E - 1]) : uu.e[E];
+            const float dnv = y == ny - 2 ? 0.5f * (uc.e[E] + qu[j].e[E - 1]) : qu[j].e[E];
+            const float lu = (((upv + dnv) + uc.e[E - 1]) + uc.e[E] - 4.f * uc.e[E]) * A.inv_h2;
+            zo.e[E] = zo.e[E - 1];
+            ao.e[E] = qa[j].e[E] + zo.e[E] * lu;
           }
           const int id = y * nx + x0;
           if (!skip) {
