@@ -34,6 +34,7 @@ for p in (os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 from _pkg import tomo  # noqa: E402
+import tomo_amd.workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell-update of one shot.  The per-cell
@@ -50,19 +51,8 @@ WORKLOADS = {
 }
 
 
-def make_workload(name, seed=3000):
-    n, steps, shots, ckpt = WORKLOADS[name]
-    nt = steps + 2
-    c = tomo.config_unit(n, nt)
-    c.checkpoint_every = ckpt
-    ii, jj = tomo.transducers_ring(n, n, shots, 0.39 * n)
-    region = (c.mask_x0, c.mask_x1, c.mask_y0, c.mask_y1)
-    fo = np.zeros(n * n, np.float32)
-    if name == "c3":
-        tomo.phantom_smooth(fo, n, n, seed, 8.0, 0.08, region)
-        tomo.phantom_inclusions(fo, n, n, seed + 1, 4, 4 * n / 128, 12 * n / 128, 0.05, 0.10, region)
-    else:
-        tomo.phantom_inclusions(fo, n, n, seed, 4, 4 * n / 128, 12 * n / 128, 0.05, 0.10, region)
+def make_workload(name):
+    c, ii, jj, fo = W.problem(name)
     return c, ii, jj, fo
 
 
@@ -85,9 +75,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--graphs", action="store_true", help="replay each gradient pass as a captured HIP graph")
+    ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows,fwd_depth,adj_depth (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--data", default="fp64", choices=["fp64", "fp32"],
+                    help="solver of the measured traces (fp32 for counter-collection runs; the timed region is the same)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 1 GiB device to device (a plain copy of known size, used to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -99,24 +93,32 @@ def main():
               "that path is not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c4":
+        run_c4(args)
+        return
     c, ii, jj, fo = make_workload(args.workload)
     n, steps, shots, ckpt = WORKLOADS[args.workload]
     e = tomo.Engine(c, device=0)
     if args.shape:
         e.set_shape(*[int(v) for v in args.shape.split(",")])
+    if args.graphs:
+        e.set_graphs(True)
     e.set_transducers(ii, jj, one_based=False)
     e.set_receivers(ii, jj, one_based=False)
     e.set_groups(1)
     e.set_truth(fo)
     e.set_field(fo)
     t = time.perf_counter()
-    meas = e.forward(fp64=True)
+    meas = e.forward(fp64=args.data == "fp64")
     t_data = time.perf_counter() - t
     e.set_field(np.zeros(fo.size, np.float32))
     e.set_measured(meas)
     print(f"[bench] workload {args.workload}: {n}x{n}, {shots} shots, {steps} steps; "
-          f"FP64 data in {t_data:.2f} s", file=sys.stderr, flush=True)
+          f"{args.data.upper()} data in {t_data:.2f} s", file=sys.stderr, flush=True)
 
+    if args.autotune:
+        shape, _ = e.autotune()
+        print(f"[bench] autotuned shape {shape}", file=sys.stderr, flush=True)
     # relaxation: a fixed scale that moves the field by at most 1e-3 in the first step
     e.gradient()
     g = np.stack([e.get_gradient(s) for s in range(shots)]).astype(np.float64).sum(0) / shots
@@ -133,7 +135,7 @@ def main():
         print(f"[bench] step {k}: eps {ep:.4f} %", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     tm = e.timing()
-    cells = tm["fwd_cells"] + tm["adj_cells"]
+    cells = tm["fwd_cells"] + tm["adj_cells"] + e.timing_graphs()["cells"]
     value = cells / elapsed / 1e9
 
     cells_per_launch = shots * n * n
@@ -154,7 +156,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded phantom; measured traces from the engine's FP64 solver)",
+        "data": f"synthetic (seeded phantom; measured traces from the engine's {args.data.upper()} solver)",
         "config": {
             "workload": f"batched plain SIRT iteration: {n}x{n} grid, {shots} ring shots, {steps} steps, "
                         f"checkpoint every {ckpt or 'none'}",
@@ -198,6 +200,75 @@ def main():
         print("[bench] copy calibration: 1 GiB device copy", file=sys.stderr, flush=True)
         b.copy_(a)
         torch.cuda.synchronize()
+    print(json.dumps(out), flush=True)
+
+
+def run_c4(args):
+    """Uneven batch: 64 phantoms of sides {128..512}, one batched launch per
+    time level and stencil shape, shapes from the autotuner per size class;
+    a step is one weighted SIRT iteration of every phantom."""
+    log = (lambda m: print("[bench]", m, file=sys.stderr, flush=True))
+    t = time.perf_counter()
+    engines = W.c4_problems(log=log)
+    t_data = time.perf_counter() - t
+    sides = W.c4_sides()
+    shapes = {}
+    for e, n in zip(engines, sides):
+        shapes[n] = e.autotune()[0]
+    log(f"data for {len(engines)} phantoms in {t_data:.1f} s; shapes per side {shapes}")
+    b = tomo.Batch(engines, graph=True)
+    b.gradient()
+    scales = []
+    for e, n in zip(engines, sides):
+        g = np.stack([e.get_gradient(s) for s in range(e.num_shots)]).astype(np.float64).sum(0) / e.num_shots
+        scales.append(float(1e-3 / max(np.abs(g).max(), 1e-30)))
+
+    def iteration():
+        b.gradient()
+        for e, sc in zip(engines, scales):
+            e.update_weighted(sc)
+
+    for _ in range(args.warmup):
+        iteration()
+    t0_cells = b.timing()["cells"]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        iteration()
+        log(f"step {k} done")
+    elapsed = time.perf_counter() - t0
+    tb = b.timing()
+    cells = tb["cells"] - t0_cells
+    eps = float(np.mean([e.error() for e in engines]))
+    per_call_ms = tb["ms"] / max(tb["calls"], 1)
+    cells_call = tb["cells"] / max(tb["calls"], 1)
+    # algorithmic bytes of a gradient call: 12 B per forward and recompute
+    # cell-update, 24 B per adjoint cell-update (one third of the updates)
+    gbs = (cells_call * (2 * FWD_BYTES + ADJ_BYTES) / 3) / (per_call_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s of FDTD step",
+        "value": round(cells / elapsed / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded phantoms; measured traces from the engine's FP64 solver)",
+        "config": {
+            "workload": "uneven batch: 64 phantoms, sides from {128,192,256,384,512}, n/4 ring shots, 8n steps, "
+                        "one weighted SIRT iteration of every phantom per step, batched launches (graph replay)",
+            "sides": sides, "shapes": {str(k): list(v) for k, v in shapes.items()},
+            "cell_updates_per_step": cells / args.steps, "mean_eps_after_steps": eps,
+            "launches_per_gradient": tb["launches"] / max(tb["calls"], 1),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "batched gradient (forward_vec, adjoint_vec, edge passes)",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": None, "avg_gradient_ms": round(per_call_ms, 3),
+        },
+        "cpu_baseline": None,
+    }
+    b.close()
+    for e in engines:
+        e.close()
     print(json.dumps(out), flush=True)
 
 

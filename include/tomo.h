@@ -244,6 +244,24 @@ int tomo_get_coverage(tomo_ctx *ctx, float *out);
 
 /* ---------------------------------------------------------------- updates */
 
+/* Ground-truth line search over a scale grid s_k = s_lo + k s_step, k =
+ * 0 .. count-1 (sirt-test-weight.cu:320-400, sirt-norm-test-weight.cu:
+ * 407-500: scales 0, 100, .., 1e6): the scale whose update (uniform MEAN, or
+ * weighted when `weighted`) gives the smallest relative error against the
+ * truth, the first one on ties.  The update is linear in s, so eps(s)^2 is a
+ * quadratic whose three FP64 coefficients locate the grid minimum exactly;
+ * the field is not changed.  Needs the gradients of all shots. */
+int tomo_line_search(tomo_ctx *ctx, int weighted, double s_lo, double s_step, int count, double *best_scale,
+                     double *best_eps);
+
+/* Per-pixel illumination weights of the field updates (C2's "per-pixel
+ * weights from sensor-coverage normalisation"; no reference counterpart):
+ * mode 1 multiplies each pixel's update by w(x) = cbar / (c(x) + eps cmax),
+ * c the coverage (tomo_enable_coverage) summed over the shots of the update
+ * (the single shot for tomo_update_group), cbar its mean over the mask and
+ * cmax its maximum; mode 0 (default) leaves the updates as they are. */
+int tomo_set_pixel_weights(tomo_ctx *ctx, int mode, double eps);
+
 /* f += scale * mask * (sum_g G_g) / (1 + f) [/ Ng]  (enum tomo_uniform_mode). */
 int tomo_update_uniform(tomo_ctx *ctx, float scale, int mode);
 
@@ -307,8 +325,59 @@ int tomo_timing_reset(tomo_ctx *ctx);
  * FP32 rounding.  No reference counterpart (its launch shapes are fixed,
  * sirt-mimo.cu:165-299). */
 int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth);
+
+/* Autotuner: times the vectorised forward and adjoint kernels of this
+ * context (its grid, shots per pass, sponge and coverage) over the shapes
+ * vec in {4, 2} x rows in {8, 16, 32} x prefetch depth in {1, 2, 4}
+ * (adjoint depth <= 2) on the context's own buffers, picks per lane width
+ * the fastest forward and adjoint settings, and keeps the lane width with
+ * the lowest gradient-pass cost (forward time x forward launches per
+ * adjoint launch + adjoint time).  The result is set as the context's
+ * shape and cached per size class for the process (force = 1 re-times).
+ * out5 (may be NULL) = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth};
+ * ms2 (may be NULL) = {forward, adjoint} ms per launch.  Invalidates stored
+ * gradients.  No reference counterpart (the reference launches one thread
+ * per cell in fixed 2-D blocks, sirt-mimo.cu:290-300). */
+int tomo_autotune(tomo_ctx *ctx, int force, int *out5, double *ms2);
+
+/* Graph replay of gradient passes (off by default).  When on, each pass of
+ * tomo_gradient (all its fills, forward steps, misfit, segment recomputes
+ * and adjoint steps: about 3 nt launches) is captured into a HIP graph on
+ * first use and replayed afterwards; a pass is captured again when any
+ * buffer, geometry, shape or coverage setting it baked in has changed.
+ * The results are bitwise those of the eager launches.  Replaces the
+ * reference's per-step launch sequence (sirt-mimo.cu:326-366) with one
+ * launch per pass. */
+int tomo_set_graphs(tomo_ctx *ctx, int on);
+/* replayed-graph time: out3 = {ms, kernel launches covered, cell updates} */
+int tomo_timing_graphs(tomo_ctx *ctx, double *out3);
 /* out5 = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth} in effect */
 int tomo_get_shape(const tomo_ctx *ctx, int *out5);
+
+/* ------------------------------------------------------------------ batch
+ * Many independent problems stepped together (uneven batches of phantoms
+ * of different grid, shot and step counts).  No reference counterpart: the
+ * reference runs one problem per process.  Every time level of every shot
+ * pass is one launch per stencil shape over the tiles of all problems
+ * active at that level (problems aligned on the absolute level k; a common
+ * checkpoint interval; shot passes in lockstep).  The contexts stay owned by
+ * the caller and must outlive the batch; each needs its shots, receivers
+ * and measured traces set, the sponge preset and the vectorised kernels
+ * (nx a multiple of 2 or 4, nx >= 8), one checkpoint interval for all.  The
+ * launch program is built at creation for the contexts' buffers and shapes
+ * as they are then (set the shapes, e.g. with tomo_autotune, before). */
+typedef struct tomo_batch tomo_batch;
+
+int tomo_batch_create(tomo_ctx *const *ctxs, int n, tomo_batch **out);
+void tomo_batch_destroy(tomo_batch *batch);
+const char *tomo_batch_last_error(const tomo_batch *batch);
+/* replay the whole program as one captured HIP graph (off by default) */
+int tomo_batch_set_graph(tomo_batch *batch, int on);
+/* the gradients of all shots of every context (as tomo_gradient on each),
+ * misfits[n] (may be NULL) per context */
+int tomo_batch_gradient(tomo_batch *batch, double *misfits);
+/* out4 = {ms, cell updates, launches, gradient calls} since creation */
+int tomo_batch_timing(const tomo_batch *batch, double *out4);
 
 /* ------------------------------------------------------------------ learned
  * Learned preconditioner.  No counterpart in the reference (which has no
@@ -351,6 +420,9 @@ int tomo_cnn_loss(tomo_cnn *net, int batch, int h, int w, const float *x, const 
  * one grid size and mask */
 int tomo_cnn_add_sample(tomo_cnn *net, tomo_ctx *ctx, float scale);
 int tomo_cnn_num_samples(const tomo_cnn *net);
+/* new targets (field - f_s) / 0.1 for samples [first, first + count), f_s the
+ * field each was taken at (e.g. field = a later iterate of the trajectory) */
+int tomo_cnn_set_targets(tomo_cnn *net, int first, int count, const float *field);
 int tomo_cnn_clear_samples(tomo_cnn *net);
 /* `steps` Adam steps (beta 0.9 / 0.999) on random batches; losses[steps] optional */
 int tomo_cnn_train(tomo_cnn *net, int steps, int batch, double lr, uint64_t seed, double *losses);

@@ -76,3 +76,15 @@ def test_generate_data_then_sirt_and_art(tmp_path):
     assert r.returncode == 0, r.stderr
     f_art = np.loadtxt(tmp_path / "art-160-fo.txt", dtype=np.float32)
     assert np.isfinite(f_art).all() and np.abs(f_art).max() > 0
+    # ART warm start then weighted SIRT (sirt-mimo-seq.cu): 7 iterations, the
+    # first 5 ART
+    r = run(["sirt-mimo-seq", "fo.txt", 160, 0, 7, 1, 20000, 0, 1], tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = [l.split() for l in r.stdout.splitlines()[2:] if l.strip()]
+    assert len(rows) >= 1 and os.path.exists(tmp_path / "sirt-norm-160-fo.txt")
+    # ground-truth line search (sirt-test-weight.cu): the first logged row of
+    # sirt-dyn-160-0.txt is iteration 1, eps 83.657, scale 22500
+    r = run(["sirt-test-weight", "fo.txt", 160, 0, 0, 1], tmp_path)
+    assert r.returncode == 0, r.stderr
+    row = [l.split() for l in r.stdout.splitlines()[2:] if l.strip()][0]
+    assert row[0] == "1" and abs(float(row[1]) - 83.657) < 0.05 and abs(float(row[3]) - 22500) <= 300

@@ -343,3 +343,36 @@ def test_data_dynamic_kat_on_engine():
         say("  ", name, "eps", eps, "log", row[1])
         assert abs(eps - row[1]) < 0.04
         assert abs(d @ fo / (d @ d) - row[3]) < 0.005 * row[3]
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_line_search_reproduces_data_dynamic_logs(weighted):
+    """The ground-truth line search (sirt-test-weight.cu / sirt-norm-test-weight.cu)
+    produced the data_dynamic logs: six iterations at G = 10 from f = 0 must
+    pick the logged scales (grid 0, 100, .., 1e6) and reach the logged errors.
+    Tolerances: eps within 0.05 + 0.02 (k - 1) points, scale within 1 % + 1 %
+    per iteration (the reference evaluated eps in FP32 and the trajectories
+    separate slowly)."""
+    import json
+    import os
+    name = "sirt-norm-dyn-10-0.txt" if weighted else "sirt-dyn-10-0.txt"
+    say("line search G=10", name)
+    kat = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "data_dynamic_kat.json")))["logs"][name]
+    fo = orc.phantom_reference()
+    e, c, bbox = ref_engine(192, 401, 10, field=fo)
+    meas = e.forward()
+    e.set_field(np.zeros(fo.size, np.float32))
+    e.set_measured(meas)
+    e.set_truth(fo)
+    for it, eps_log, _, scale_log in kat:
+        e.gradient()
+        scale, eps_pred = e.line_search(weighted=weighted)
+        if weighted:
+            e.update_weighted(scale)
+        else:
+            e.update_uniform(scale, tomo.UPDATE_MEAN)
+        eps = e.error()
+        say(f"  iter {it}: scale {scale:.0f} (log {scale_log:.0f}) eps {eps:.4f} (log {eps_log:.4f}, predicted {eps_pred:.4f})")
+        assert abs(eps - eps_pred) < 1e-3
+        assert abs(eps - eps_log) <= 0.05 + 0.02 * (it - 1)
+        assert abs(scale - scale_log) <= (0.01 + 0.01 * it) * scale_log

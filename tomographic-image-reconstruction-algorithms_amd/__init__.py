@@ -108,6 +108,9 @@ SIGNATURES = {
     "tomo_get_gradient": (_int, [_vp, _int, _f32p]),
     "tomo_enable_coverage": (_int, [_vp, _int]),
     "tomo_get_coverage": (_int, [_vp, _f32p]),
+    "tomo_set_pixel_weights": (_int, [_vp, _int, ctypes.c_double]),
+    "tomo_line_search": (_int, [_vp, _int, ctypes.c_double, ctypes.c_double, _int,
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "tomo_update_uniform": (_int, [_vp, ctypes.c_float, _int]),
     "tomo_update_weighted": (_int, [_vp, ctypes.c_float, _vp]),
     "tomo_update_group": (_int, [_vp, _int, ctypes.c_float]),
@@ -121,6 +124,16 @@ SIGNATURES = {
     "tomo_timing_reset": (_int, [_vp]),
     "tomo_set_shape": (_int, [_vp, _int, _int, _int, _int, _int]),
     "tomo_get_shape": (_int, [_vp, _i32p]),
+    "tomo_set_graphs": (_int, [_vp, _int]),
+    "tomo_autotune": (_int, [_vp, _int, _i32p, _f64p]),
+    "tomo_timing_graphs": (_int, [_vp, _f64p]),
+    # batch of problems
+    "tomo_batch_create": (_int, [ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp)]),
+    "tomo_batch_destroy": (None, [_vp]),
+    "tomo_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "tomo_batch_set_graph": (_int, [_vp, _int]),
+    "tomo_batch_gradient": (_int, [_vp, _vp]),
+    "tomo_batch_timing": (_int, [_vp, _f64p]),
     # learned preconditioner
     "tomo_cnn_num_params": (_int, []),
     "tomo_cnn_create": (_int, [_int, ctypes.POINTER(_vp)]),
@@ -134,6 +147,7 @@ SIGNATURES = {
     "tomo_cnn_loss": (_int, [_vp, _int, _int, _int, _f32p, _f32p, _i32p, ctypes.POINTER(ctypes.c_double), _vp]),
     "tomo_cnn_add_sample": (_int, [_vp, _vp, ctypes.c_float]),
     "tomo_cnn_num_samples": (_int, [_vp]),
+    "tomo_cnn_set_targets": (_int, [_vp, _int, _int, _f32p]),
     "tomo_cnn_clear_samples": (_int, [_vp]),
     "tomo_cnn_train": (_int, [_vp, _int, _int, ctypes.c_double, ctypes.c_uint64, _vp]),
     "tomo_learned_update": (_int, [_vp, _vp, ctypes.c_float, ctypes.c_float]),
@@ -373,6 +387,18 @@ class Engine:
         return out
 
     # updates
+    def line_search(self, weighted=False, s_lo=0.0, s_step=100.0, count=10001):
+        """Ground-truth best scale on the grid s_lo + k s_step; returns (scale, eps)."""
+        sc = ctypes.c_double(0.0)
+        ep = ctypes.c_double(0.0)
+        self._c(lib().tomo_line_search(self._ctx, int(bool(weighted)), float(s_lo), float(s_step), int(count),
+                                       ctypes.byref(sc), ctypes.byref(ep)))
+        return sc.value, ep.value
+
+    def set_pixel_weights(self, mode=1, eps=0.05):
+        """Per-pixel illumination weights of the updates (1: coverage normalisation, 0: off)."""
+        self._c(lib().tomo_set_pixel_weights(self._ctx, int(mode), float(eps)))
+
     def update_uniform(self, scale, mode=UPDATE_MEAN):
         self._c(lib().tomo_update_uniform(self._ctx, float(scale), int(mode)))
 
@@ -414,6 +440,22 @@ class Engine:
     def timing_reset(self):
         self._c(lib().tomo_timing_reset(self._ctx))
 
+    def timing_graphs(self) -> dict:
+        out = np.zeros(3, np.float64)
+        self._c(lib().tomo_timing_graphs(self._ctx, out))
+        return {"ms": out[0], "launches": int(out[1]), "cells": out[2]}
+
+    def autotune(self, force=False):
+        """Time the stencil shapes on this context and keep the fastest; returns (shape, (fwd_ms, adj_ms))."""
+        out = np.zeros(5, np.int32)
+        ms = np.zeros(2, np.float64)
+        self._c(lib().tomo_autotune(self._ctx, int(bool(force)), out, ms))
+        return tuple(int(v) for v in out), (float(ms[0]), float(ms[1]))
+
+    def set_graphs(self, on=True):
+        """Capture each gradient pass into a HIP graph and replay it."""
+        self._c(lib().tomo_set_graphs(self._ctx, int(bool(on))))
+
     def set_shape(self, vec=0, fwd_rows=0, adj_rows=0, fwd_depth=0, adj_depth=0):
         self._c(lib().tomo_set_shape(self._ctx, int(vec), int(fwd_rows), int(adj_rows), int(fwd_depth),
                                      int(adj_depth)))
@@ -422,6 +464,58 @@ class Engine:
         out = np.zeros(5, np.int32)
         self._c(lib().tomo_get_shape(self._ctx, out))
         return tuple(int(v) for v in out)
+
+
+# --------------------------------------------------------------- batch
+
+class Batch:
+    """Several engines (problems of any sizes) stepped together, one launch
+    per time level and stencil shape.  The engines must outlive the batch."""
+
+    def __init__(self, engines, graph: bool = False):
+        self.engines = list(engines)
+        arr = (_vp * len(self.engines))(*[e._ctx for e in self.engines])
+        self._b = _vp()
+        st = lib().tomo_batch_create(arr, len(self.engines), ctypes.byref(self._b))
+        if st < 0:
+            msg = _STATUS.get(st, str(st))
+            detail = lib().tomo_last_error(None)
+            raise TomoError(msg + (": " + detail.decode(errors="replace") if detail else ""))
+        if graph:
+            self._c(lib().tomo_batch_set_graph(self._b, 1))
+
+    def _c(self, status):
+        if status < 0:
+            detail = lib().tomo_batch_last_error(self._b)
+            raise TomoError(_STATUS.get(status, str(status)) + (": " + detail.decode(errors="replace") if detail else ""))
+        return status
+
+    def gradient(self):
+        mis = np.zeros(len(self.engines), np.float64)
+        self._c(lib().tomo_batch_gradient(self._b, _ptr(mis)))
+        return mis
+
+    def timing(self) -> dict:
+        out = np.zeros(4, np.float64)
+        self._c(lib().tomo_batch_timing(self._b, out))
+        return {"ms": out[0], "cells": out[1], "launches": int(out[2]), "calls": int(out[3])}
+
+    def close(self):
+        if self._b:
+            lib().tomo_batch_destroy(self._b)
+            self._b = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # ------------------------------------------------------- learned preconditioner
@@ -502,6 +596,10 @@ class Preconditioner:
 
     def num_samples(self) -> int:
         return int(lib().tomo_cnn_num_samples(self._net))
+
+    def set_targets(self, first: int, count: int, field):
+        """Targets (field - f_s) / 0.1 for samples [first, first + count)."""
+        self._c(lib().tomo_cnn_set_targets(self._net, int(first), int(count), _f32(field)))
 
     def clear_samples(self):
         self._c(lib().tomo_cnn_clear_samples(self._net))

@@ -474,6 +474,13 @@ __global__ void learned_update_kernel(float *f, const float *y, const float *dir
     f[i] += alpha * kFieldUnit * y[i] + fmaxf(1.f - alpha, 0.f) * scale * dir[i];
 }
 
+// target[s][p] = (field[p] - f_s[p]) / unit for samples s of a block
+__global__ void retarget_kernel(const float *field, const float *fs, float *target, int count, size_t n2) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n2 * (size_t)count) return;
+  target[i] = (field[i % n2] - fs[i]) / kFieldUnit;
+}
+
 __global__ void copy_f32_kernel(const float *src, float *dst, size_t n) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -516,7 +523,8 @@ struct tomo_cnn {
   int sH = 0, sW = 0, nsamples = 0, cap_samples = 0;
   Mask smask{0, 0, 0, 0};
   __bf16 *d_sx = nullptr;   // [N][H][W][8]
-  float *d_st = nullptr;    // [N][H][W]
+  float *d_st = nullptr;    // [N][H][W]  targets
+  float *d_sf = nullptr;    // [N][H][W]  field of each sample (for new targets)
   float *d_batch_t = nullptr;
   // engine-side scratch for inference / sampling
   float *d_dir = nullptr, *d_cov = nullptr, *d_fprev = nullptr, *d_dprev = nullptr;
@@ -798,7 +806,7 @@ void tomo_cnn_destroy(tomo_cnn *n) {
   for (int l = 0; l < kLayers; ++l) { fr(n->d_wf[l]); fr(n->d_wb[l]); fr(n->d_bias[l]); }
   fr(n->d_x8); fr(n->d_y); fr(n->d_g8); fr(n->d_ga); fr(n->d_gb); fr(n->d_part); fr(n->d_tmp3); fr(n->d_batch_t);
   for (int l = 0; l < kLayers - 1; ++l) fr(n->d_act[l]);
-  fr(n->d_sx); fr(n->d_st);
+  fr(n->d_sx); fr(n->d_st); fr(n->d_sf);
   fr(n->d_dir); fr(n->d_cov); fr(n->d_fprev); fr(n->d_dprev);
   if (n->stream) (void)hipStreamDestroy(n->stream);
   delete n;
@@ -918,23 +926,45 @@ int tomo_cnn_add_sample(tomo_cnn *n, tomo_ctx *c, float scale) {
   if (n->nsamples == n->cap_samples) {
     const int cap = std::max(16, n->cap_samples * 2);
     __bf16 *sx = nullptr;
-    float *st = nullptr;
-    if ((rc = nalloc(n, &sx, (size_t)cap * n2 * 8)) || (rc = nalloc(n, &st, (size_t)cap * n2))) return rc;
+    float *st = nullptr, *sf = nullptr;
+    if ((rc = nalloc(n, &sx, (size_t)cap * n2 * 8)) || (rc = nalloc(n, &st, (size_t)cap * n2)) ||
+        (rc = nalloc(n, &sf, (size_t)cap * n2)))
+      return rc;
     if (n->nsamples) {
       CNNCHK(n, hipMemcpy(sx, n->d_sx, sizeof(__bf16) * n->nsamples * n2 * 8, hipMemcpyDeviceToDevice));
       CNNCHK(n, hipMemcpy(st, n->d_st, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
+      CNNCHK(n, hipMemcpy(sf, n->d_sf, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
     }
-    if (n->d_sx) hipFree(n->d_sx);
-    if (n->d_st) hipFree(n->d_st);
+    if (n->d_sx) (void)hipFree(n->d_sx);
+    if (n->d_st) (void)hipFree(n->d_st);
+    if (n->d_sf) (void)hipFree(n->d_sf);
     n->d_sx = sx;
     n->d_st = st;
+    n->d_sf = sf;
     n->cap_samples = cap;
   }
   if ((rc = engine_input(n, c, scale, n->d_st + (size_t)n->nsamples * n2))) return rc;
   CNNCHK(n, hipMemcpyAsync(n->d_sx + (size_t)n->nsamples * n2 * 8, n->d_x8, sizeof(__bf16) * n2 * 8,
                            hipMemcpyDeviceToDevice, c->stream));
+  CNNCHK(n, hipMemcpyAsync(n->d_sf + (size_t)n->nsamples * n2, c->d_f, sizeof(float) * n2, hipMemcpyDeviceToDevice,
+                           c->stream));
   CNNCHK(n, hipStreamSynchronize(c->stream));
   n->nsamples += 1;
+  return TOMO_OK;
+}
+
+int tomo_cnn_set_targets(tomo_cnn *n, int first, int count, const float *field) {
+  if (!n || !field || first < 0 || count < 0 || first + count > n->nsamples) return TOMO_ERR_ARG;
+  if (count == 0) return TOMO_OK;
+  CNNCHK(n, hipSetDevice(n->device));
+  const size_t n2 = (size_t)n->sH * n->sW;
+  int rc;
+  if ((rc = ensure_tmp(n, n2))) return rc;
+  CNNCHK(n, hipMemcpyAsync(n->d_tmp3, field, sizeof(float) * n2, hipMemcpyHostToDevice, n->stream));
+  hipLaunchKernelGGL(retarget_kernel, dim3(blocks_for(n2 * count)), dim3(256), 0, n->stream, (const float *)n->d_tmp3,
+                     (const float *)(n->d_sf + (size_t)first * n2), n->d_st + (size_t)first * n2, count, n2);
+  CNNCHK(n, hipGetLastError());
+  CNNCHK(n, hipStreamSynchronize(n->stream));
   return TOMO_OK;
 }
 
@@ -1037,12 +1067,16 @@ int tomo_learned_iteration(tomo_ctx *c, tomo_cnn *n, float alpha, float scale, i
   if (weighted) {
     const int pb = weighted_part_blocks((int)c->n2);
     if ((rc = ensure_scratch(c, (size_t)c->n2 + (size_t)M * pb + (size_t)M + 1))) return rc;
+    const float *pw = nullptr;
+    if ((rc = pixel_weights(c, 0, M, &pw))) return rc;  // before the scratch below is taken
     double *mean = c->d_scratch, *part = mean + c->n2, *w = part + (size_t)M * pb;
     HIPCHK(c, launch_weighted(n->d_fprev, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)scale, mean, part, pb, w,
-                              c->stream));
+                              c->stream, pw));
   } else {
+    const float *pw = nullptr;
+    if ((rc = pixel_weights(c, 0, M, &pw))) return rc;
     HIPCHK(c, launch_update_uniform(n->d_fprev, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)scale, (double)M,
-                                    c->stream));
+                                    c->stream, pw));
   }
   // the learned candidate, checked by the data misfit before it is kept
   if ((rc = tomo_learned_update(c, n, n->ls_alpha, scale))) return rc;

@@ -13,9 +13,18 @@
 
 struct TimedSpan {
   hipEvent_t a, b;
-  int kind;          // 0 forward, 1 adjoint
+  int kind;          // 0 forward, 1 adjoint, 2 graph replay
   double launches;
   double cells;
+};
+
+// One captured gradient pass (shots [p0, p0 + np) in passes of P) and the
+// pointers and parameters its launches baked in.
+struct GraphEntry {
+  int p0 = 0, np = 0, P = 0;
+  std::vector<uintptr_t> sig;
+  hipGraphExec_t exec = nullptr;
+  double launches[2] = {0, 0}, cells[2] = {0, 0};
 };
 
 struct tomo_ctx {
@@ -59,7 +68,16 @@ struct tomo_ctx {
   // timing
   std::vector<hipEvent_t> ev_free;
   std::vector<TimedSpan> spans;
-  double t_ms[2] = {0, 0}, t_launch[2] = {0, 0}, t_cells[2] = {0, 0};
+  double t_ms[3] = {0, 0, 0}, t_launch[3] = {0, 0, 0}, t_cells[3] = {0, 0, 0};
+  // per-pixel illumination weights of the updates (0 off, 1 coverage)
+  int pixw_mode = 0;
+  double pixw_eps = 0.05;
+  float *d_pixw = nullptr;
+  // graph replay of gradient passes
+  bool use_graphs = false;
+  bool capturing = false;
+  double cap_launches[2] = {0, 0}, cap_cells[2] = {0, 0};
+  std::vector<GraphEntry> graphs;
 };
 
 namespace tomo_internal {
@@ -78,7 +96,7 @@ inline int fail(tomo_ctx *c, int code, const std::string &msg) {
 template <typename T>
 inline int dalloc(tomo_ctx *c, T **p, size_t n) {
   if (*p) {
-    hipFree(*p);
+    (void)hipFree(*p);
     *p = nullptr;
   }
   if (n == 0) return TOMO_OK;
@@ -93,7 +111,7 @@ inline int dalloc(tomo_ctx *c, T **p, size_t n) {
 
 template <typename T>
 inline void dfree(T *&p) {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
 }
 
@@ -102,6 +120,9 @@ int need_shots(tomo_ctx *c);
 int need_all_gradients(tomo_ctx *c);
 int upload_coefficients(tomo_ctx *c);  // per-cell coefficients from the current field
 int ensure_scratch(tomo_ctx *c, size_t n);
+// per-pixel update weights from the coverage of shots [g0, g0 + ng) into
+// d_pixw (*w = d_pixw), or *w = nullptr when the weights are off
+int pixel_weights(tomo_ctx *c, int g0, int ng, const float **w);
 inline tomo::Mask ctx_mask(const tomo_ctx *c) {
   return tomo::Mask{c->cfg.mask_x0, c->cfg.mask_x1, c->cfg.mask_y0, c->cfg.mask_y1};
 }

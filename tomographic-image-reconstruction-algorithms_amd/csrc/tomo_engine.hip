@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -84,17 +85,32 @@ hipEvent_t take_event(tomo_ctx *c) {
 }
 
 void span_begin(tomo_ctx *c, TimedSpan &s, int kind) {
-  s.a = take_event(c);
-  s.b = take_event(c);
   s.kind = kind;
   s.launches = 0;
   s.cells = 0;
+  if (c->capturing) {  // no events inside a captured graph: counts only
+    s.a = s.b = nullptr;
+    return;
+  }
+  s.a = take_event(c);
+  s.b = take_event(c);
   if (s.a) hipEventRecord(s.a, c->stream);
 }
 
 void span_end(tomo_ctx *c, TimedSpan &s) {
+  if (c->capturing) {
+    c->cap_launches[s.kind] += s.launches;
+    c->cap_cells[s.kind] += s.cells;
+    return;
+  }
   if (s.b) hipEventRecord(s.b, c->stream);
   c->spans.push_back(s);
+}
+
+void graphs_clear(tomo_ctx *c) {
+  for (auto &g : c->graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
 }
 
 // after a stream sync: fold recorded spans into the totals
@@ -466,6 +482,71 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   return TOMO_OK;
 }
 
+// every pointer and parameter a captured gradient pass depends on
+std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
+  const StencilShape sh = stencil_shape(c);
+  const uintptr_t v[] = {(uintptr_t)c->d_ckpt, (uintptr_t)c->d_seg, (uintptr_t)c->d_z, (uintptr_t)c->d_rr,
+                         (uintptr_t)c->d_meas, (uintptr_t)c->d_pw, (uintptr_t)c->d_f, (uintptr_t)c->d_grad,
+                         (uintptr_t)c->d_cov, (uintptr_t)c->d_misfit, (uintptr_t)c->d_ring, (uintptr_t)c->nring,
+                         (uintptr_t)c->d_bbox, (uintptr_t)c->d_cellmap, (uintptr_t)c->d_anchor,
+                         (uintptr_t)c->d_rowflag, (uintptr_t)c->d_spa, (uintptr_t)c->d_spab,
+                         (uintptr_t)c->coverage, (uintptr_t)c->nshots, (uintptr_t)c->nr, (uintptr_t)sh.vec,
+                         (uintptr_t)sh.fwd_rows, (uintptr_t)sh.adj_rows, (uintptr_t)sh.fwd_depth,
+                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every};
+  return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
+}
+
+// gradient pass through a captured graph: captured on first use (and again
+// whenever a pointer or parameter it baked in changed), then replayed
+int gradient_pass_graph(tomo_ctx *c, int p0, int np, int P) {
+  const std::vector<uintptr_t> sig = graph_signature(c);
+  GraphEntry *ge = nullptr;
+  for (auto &g : c->graphs)
+    if (g.p0 == p0 && g.np == np && g.P == P) ge = &g;
+  if (ge && ge->sig != sig) {
+    hipGraphExecDestroy(ge->exec);
+    ge->exec = nullptr;
+  }
+  if (!ge || !ge->exec) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    c->cap_launches[0] = c->cap_launches[1] = c->cap_cells[0] = c->cap_cells[1] = 0;
+    int rc = gradient_pass(c, p0, np, P);
+    c->capturing = false;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      return rc ? rc : fail(c, TOMO_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(c, TOMO_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    if (!ge) {
+      c->graphs.emplace_back();
+      ge = &c->graphs.back();
+      ge->p0 = p0;
+      ge->np = np;
+      ge->P = P;
+    }
+    ge->exec = exec;
+    ge->sig = sig;
+    for (int k = 0; k < 2; ++k) {
+      ge->launches[k] = c->cap_launches[k];
+      ge->cells[k] = c->cap_cells[k];
+    }
+  }
+  TimedSpan s;
+  span_begin(c, s, 2);
+  HIPCHK(c, hipGraphLaunch(ge->exec, c->stream));
+  s.launches = ge->launches[0] + ge->launches[1];
+  s.cells = ge->cells[0] + ge->cells[1];
+  span_end(c, s);
+  return TOMO_OK;
+}
+
 }  // namespace
 
 namespace tomo_internal {
@@ -507,6 +588,43 @@ int need_all_gradients(tomo_ctx *c) {
     if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradients of all shots are required");
   return TOMO_OK;
 }
+
+int pixel_weights(tomo_ctx *c, int g0, int ng, const float **w) {
+  *w = nullptr;
+  if (c->pixw_mode == 0) return TOMO_OK;
+  if (!c->coverage || !c->d_cov) return fail(c, TOMO_ERR_STATE, "coverage weights need tomo_enable_coverage");
+  int rc;
+  if (!c->d_pixw && (rc = dalloc(c, &c->d_pixw, (size_t)c->n2))) return rc;
+  if ((rc = ensure_scratch(c, 2))) return rc;
+  HIPCHK(c, launch_pixel_weights(c->d_cov + (size_t)g0 * c->n2, ng, c->nx, c->ny, ctx_mask(c), c->pixw_eps,
+                                 c->d_scratch, c->d_pixw, c->stream));
+  *w = c->d_pixw;
+  return TOMO_OK;
+}
+
+}  // namespace tomo_internal
+
+namespace tomo_internal {
+
+// entry points of the batched stepping (tomo_batch.hip)
+StencilShape context_shape(const tomo_ctx *c) { return stencil_shape(c); }
+float *level_pass1(tomo_ctx *c, int P, int k) { return pass1_level(c, P, k, true); }
+float *level_segment(tomo_ctx *c, int P, int m, int k) { return segment_level(c, P, m, k); }
+FwdArgs<float> forward_args(tomo_ctx *c, int shot0) {
+  FwdArgs<float> a = fwd_args<float>(c, shot0, c->d_pw, c->d_spa, c->d_spab);
+  a.ring = c->d_ring;
+  a.nring = c->nring;
+  return a;
+}
+AdjArgs adjoint_args(tomo_ctx *c, int shot0) {
+  AdjArgs a = adj_args(c, shot0);
+  a.ring = c->d_ring;
+  a.nring = c->nring;
+  return a;
+}
+float pulse_h2_f32(const tomo_ctx *c, int k) { return pulse_h2<float>(c, k); }
+int wavefields(tomo_ctx *c, int P) { return ensure_wavefields(c, P, true); }
+int shots_per_pass(const tomo_ctx *c) { return pass_shots(c); }
 
 }  // namespace tomo_internal
 
@@ -601,10 +719,11 @@ void tomo_destroy(tomo_ctx *c) {
     if (s.b) hipEventDestroy(s.b);
   }
   for (auto e : c->ev_free) hipEventDestroy(e);
+  graphs_clear(c);
   dfree(c->d_f); dfree(c->d_fo); dfree(c->d_pw); dfree(c->d_pw64);
   dfree(c->d_cellmap); dfree(c->d_anchor); dfree(c->d_bbox); dfree(c->d_ring); dfree(c->d_rowflag);
   dfree(c->d_spa); dfree(c->d_spab); dfree(c->d_spa64); dfree(c->d_spab64);
-  dfree(c->d_meas); dfree(c->d_rr); dfree(c->d_grad); dfree(c->d_cov); dfree(c->d_misfit);
+  dfree(c->d_meas); dfree(c->d_rr); dfree(c->d_grad); dfree(c->d_cov); dfree(c->d_misfit); dfree(c->d_pixw);
   dfree(c->d_scratch); dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_z); dfree(c->d_ring64);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -797,7 +916,7 @@ int tomo_gradient(tomo_ctx *c, int g0, int ng, double *misfit) {
   if ((rc = ensure_wavefields(c, P, true))) return rc;
   for (int p0 = g0; p0 < g0 + ng; p0 += P) {
     const int np = std::min(P, g0 + ng - p0);
-    if ((rc = gradient_pass(c, p0, np, P))) return rc;
+    if ((rc = c->use_graphs ? gradient_pass_graph(c, p0, np, P) : gradient_pass(c, p0, np, P))) return rc;
   }
   std::vector<double> mis(ng);
   HIPCHK(c, hipMemcpyAsync(mis.data(), c->d_misfit + g0, sizeof(double) * ng, hipMemcpyDeviceToHost, c->stream));
@@ -856,8 +975,10 @@ int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
   if (rc) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
   const double div = mode == TOMO_UPDATE_MEAN ? (double)c->nshots : 1.0;
+  const float *pw = nullptr;
+  if ((rc = pixel_weights(c, 0, c->nshots, &pw))) return rc;
   HIPCHK(c, launch_update_uniform(c->d_f, c->d_grad, c->nshots, c->nx, c->ny, ctx_mask(c), (double)scale, div,
-                                  c->stream));
+                                  c->stream, pw));
   if ((rc = upload_coefficients(c))) return rc;
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -873,8 +994,21 @@ int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
   const size_t need = (size_t)c->n2 + (size_t)M * pb + (size_t)M + 1;
   if ((rc = ensure_scratch(c, need))) return rc;
   double *mean = c->d_scratch, *part = mean + c->n2, *w = part + (size_t)M * pb;
+  const float *pw = nullptr;
+  if (c->pixw_mode) {
+    // the weights' own scratch: a separate slot past the weighted-update scratch
+    if ((rc = ensure_scratch(c, need + 2))) return rc;
+    mean = c->d_scratch;
+    part = mean + c->n2;
+    w = part + (size_t)M * pb;
+    if (!c->coverage || !c->d_cov) return fail(c, TOMO_ERR_STATE, "coverage weights need tomo_enable_coverage");
+    if (!c->d_pixw && (rc = dalloc(c, &c->d_pixw, (size_t)c->n2))) return rc;
+    HIPCHK(c, launch_pixel_weights(c->d_cov, M, c->nx, c->ny, ctx_mask(c), c->pixw_eps, c->d_scratch + need,
+                                   c->d_pixw, c->stream));
+    pw = c->d_pixw;
+  }
   HIPCHK(c, launch_weighted(c->d_f, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)scale, mean, part, pb, w,
-                            c->stream));
+                            c->stream, pw));
   if ((rc = upload_coefficients(c))) return rc;
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
   if (weights_out)
@@ -889,8 +1023,10 @@ int tomo_update_group(tomo_ctx *c, int g, float scale) {
   if ((rc = need_shots(c))) return rc;
   if (g < 0 || g >= c->nshots) return fail(c, TOMO_ERR_ARG, "bad shot index");
   if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradient of this shot not computed");
+  const float *pw = nullptr;
+  if ((rc = pixel_weights(c, g, 1, &pw))) return rc;
   HIPCHK(c, launch_update_group(c->d_f, c->d_grad + (size_t)g * c->n2, c->nx, c->ny, ctx_mask(c), (double)scale,
-                                c->stream));
+                                c->stream, pw));
   if ((rc = upload_coefficients(c))) return rc;
   // the field changed: every stored gradient is stale
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);
@@ -964,8 +1100,238 @@ int tomo_timing(tomo_ctx *c, double *out6) {
 
 int tomo_timing_reset(tomo_ctx *c) {
   if (!c) return TOMO_ERR_ARG;
-  for (int i = 0; i < 2; ++i) c->t_ms[i] = c->t_launch[i] = c->t_cells[i] = 0;
+  for (int i = 0; i < 3; ++i) c->t_ms[i] = c->t_launch[i] = c->t_cells[i] = 0;
   return TOMO_OK;
+}
+
+int tomo_line_search(tomo_ctx *c, int weighted, double s_lo, double s_step, int count, double *best_scale,
+                     double *best_eps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
+  if (!c->have_truth) return fail(c, TOMO_ERR_STATE, "no ground truth: call tomo_set_truth first");
+  if (count <= 0 || !(s_step > 0.0) || !best_scale) return fail(c, TOMO_ERR_ARG, "bad scale grid");
+  const int M = c->nshots;
+  const int pb = weighted_part_blocks((int)c->n2);
+  // scratch: [weighted mean n2][partials M pb][weights M][pad][direction n2][sums 4][pixel stats 2]
+  const size_t off_w = (size_t)c->n2 + (size_t)M * pb;
+  const size_t off_d = off_w + (size_t)M + 1;
+  const size_t off_s = off_d + (size_t)c->n2;
+  if ((rc = ensure_scratch(c, off_s + 6))) return rc;
+  const float *pw = nullptr;
+  if (c->pixw_mode) {
+    if (!c->coverage || !c->d_cov) return fail(c, TOMO_ERR_STATE, "coverage weights need tomo_enable_coverage");
+    if (!c->d_pixw && (rc = dalloc(c, &c->d_pixw, (size_t)c->n2))) return rc;
+    HIPCHK(c, launch_pixel_weights(c->d_cov, M, c->nx, c->ny, ctx_mask(c), c->pixw_eps, c->d_scratch + off_s + 4,
+                                   c->d_pixw, c->stream));
+    pw = c->d_pixw;
+  }
+  const double *w = nullptr;
+  if (weighted) {
+    HIPCHK(c, launch_weights_only(c->d_f, c->d_grad, M, c->nx, c->ny, c->d_scratch, c->d_scratch + c->n2, pb,
+                                  c->d_scratch + off_w, c->stream));
+    w = c->d_scratch + off_w;
+  }
+  HIPCHK(c, launch_error_quadratic(c->d_f, c->d_fo, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)M, w, pw,
+                                   c->d_scratch + off_d, c->d_scratch + off_s, c->stream));
+  double q[4];
+  HIPCHK(c, hipMemcpyAsync(q, c->d_scratch + off_s, sizeof(q), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // eps(s)^2 is the quadratic a + 2 b s + c s^2 over the scale grid
+  auto err2 = [&](int k) {
+    const double s = s_lo + s_step * k;
+    return q[0] + 2.0 * q[1] * s + q[2] * s * s;
+  };
+  int kbest = 0;
+  if (q[2] > 0.0) {
+    const double ks = (-q[1] / q[2] - s_lo) / s_step;
+    const int k0 = (int)std::max(0.0, std::min((double)(count - 1), std::floor(ks)));
+    kbest = k0;
+    for (int k = std::max(0, k0 - 1); k <= std::min(count - 1, k0 + 2); ++k)
+      if (err2(k) < err2(kbest)) kbest = k;
+    if (err2(0) <= err2(kbest)) kbest = 0;  // the first grid point wins ties
+  }
+  *best_scale = s_lo + s_step * kbest;
+  if (best_eps) *best_eps = 100.0 * std::sqrt(std::max(err2(kbest), 0.0)) / std::sqrt(q[3]);
+  return TOMO_OK;
+}
+
+int tomo_set_pixel_weights(tomo_ctx *c, int mode, double eps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (mode != 0 && mode != 1) return fail(c, TOMO_ERR_ARG, "pixel weight mode must be 0 or 1");
+  if (!(eps > 0.0)) return fail(c, TOMO_ERR_ARG, "eps must be positive");
+  c->pixw_mode = mode;
+  c->pixw_eps = eps;
+  return TOMO_OK;
+}
+
+int tomo_timing_graphs(tomo_ctx *c, double *out3) {
+  if (!c || !out3) return TOMO_ERR_ARG;
+  out3[0] = c->t_ms[2];
+  out3[1] = c->t_launch[2];
+  out3[2] = c->t_cells[2];
+  return TOMO_OK;
+}
+
+int tomo_set_graphs(tomo_ctx *c, int on) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  c->use_graphs = on != 0;
+  if (!c->use_graphs) graphs_clear(c);
+  return TOMO_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shapes chosen by the autotuner, per size class (grid, shots per pass,
+// sponge, coverage), shared by every context of the process.
+struct TuneEntry {
+  int nx, ny, P, sponge, cov;
+  StencilShape shape;
+  double fwd_ms, adj_ms;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune;
+
+// average ms of `reps` launches of fn on the context stream (min of 2 trials)
+template <typename F>
+int time_launches(tomo_ctx *c, int reps, F fn, double *ms) {
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return fail(c, TOMO_ERR_HIP, "hipEventCreate failed");
+  double best = 1e30;
+  int rc = TOMO_OK;
+  for (int trial = 0; trial < 3 && !rc; ++trial) {
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps && !rc; ++r) rc = fn();
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, a, b));
+    if (trial > 0) best = std::min(best, (double)t / reps);  // trial 0 warms up
+  }
+  c->ev_free.push_back(a);
+  c->ev_free.push_back(b);
+  *ms = best;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c))) return rc;
+  const int P = pass_shots(c);
+  const int sponge = c->cfg.sponge_width > 0;
+  const int cov = c->coverage ? 1 : 0;
+  auto finish = [&](const TuneEntry &t) {
+    c->shape_req = t.shape;
+    c->shape_req.rows = c->shape_req.depth = 0;
+    c->shape_req.ring = nullptr;
+    c->shape_req.nring = 0;
+    if (out5) {
+      out5[0] = t.shape.vec;
+      out5[1] = t.shape.fwd_rows;
+      out5[2] = t.shape.adj_rows;
+      out5[3] = t.shape.fwd_depth;
+      out5[4] = t.shape.adj_depth;
+    }
+    if (ms2) {
+      ms2[0] = t.fwd_ms;
+      ms2[1] = t.adj_ms;
+    }
+    return TOMO_OK;
+  };
+  if (!force) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &t : g_tune)
+      if (t.nx == c->nx && t.ny == c->ny && t.P == P && t.sponge == sponge && t.cov == cov) return finish(t);
+  }
+  if ((rc = ensure_wavefields(c, P, true))) return rc;
+  const size_t lvl = (size_t)P * c->n2;
+  float *Z0 = c->d_z, *Z1 = c->d_z + lvl, *Z2 = c->d_z + 2 * lvl;
+  const StencilShape keep = c->shape_req;
+  const int mid = c->nt / 2;
+  FwdArgs<float> fa = fwd_args<float>(c, 0, c->d_pw, c->d_spa, c->d_spab);
+  AdjArgs aa = adj_args(c, 0);
+  aa.z1 = Z1;
+  aa.z2 = Z2;
+  aa.z0 = Z0;
+  aa.u = pass1_level(c, P, 0, true);
+  aa.level = mid;
+  const int reps = std::max(4, std::min(32, (int)(4.0e8 / ((double)lvl + 1.0))));
+  TuneEntry best{c->nx, c->ny, P, sponge, cov, StencilShape{}, 1e30, 1e30};
+  double best_cost = 1e30;
+  const int fw = full_history(c) ? 1 : 2;  // forward launches per adjoint launch in a gradient pass
+  for (int vec : {4, 2}) {
+    if (c->nx % vec != 0 || c->nx < 4 * vec) continue;
+    double bf = 1e30, ba = 1e30;
+    int brf = 0, bdf = 0, bra = 0, bda = 0;
+    for (int rows : {8, 16, 32}) {
+      if (rows > c->ny) continue;
+      for (int depth : {1, 2, 4}) {
+        c->shape_req = StencilShape{};
+        c->shape_req.vec = vec;
+        c->shape_req.fwd_rows = rows;
+        c->shape_req.fwd_depth = depth;
+        c->shape_req.adj_rows = rows;
+        c->shape_req.adj_depth = depth;
+        double tf = 0.0, ta = 0.0;
+        rc = time_launches(
+            c, reps, [&]() -> int { return forward_step<float>(c, fa, Z0, Z1, Z2, mid, P, false, nullptr, nullptr); },
+            &tf);
+        if (!rc && depth <= 2) {
+          StencilShape sh = stencil_shape(c);
+          sh.rows = sh.adj_rows;
+          sh.depth = sh.adj_depth;
+          rc = time_launches(c, reps, [&]() -> int {
+            HIPCHK(c, launch_adjoint_step(aa, P, sh, c->stream));
+            return TOMO_OK;
+          }, &ta);
+        } else {
+          ta = 1e30;
+        }
+        if (rc) {
+          c->shape_req = keep;
+          return rc;
+        }
+        if (tf < bf) { bf = tf; brf = rows; bdf = depth; }
+        if (ta < ba) { ba = ta; bra = rows; bda = depth; }
+      }
+    }
+    const double cost = fw * bf + ba;
+    if (brf && bra && cost < best_cost) {
+      best_cost = cost;
+      best.shape = StencilShape{};
+      best.shape.vec = vec;
+      best.shape.fwd_rows = brf;
+      best.shape.fwd_depth = bdf;
+      best.shape.adj_rows = bra;
+      best.shape.adj_depth = bda;
+      best.fwd_ms = bf;
+      best.adj_ms = ba;
+    }
+  }
+  c->shape_req = keep;
+  // the timing launches wrote the gradient accumulators
+  std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);
+  if (best.shape.vec == 0) return TOMO_OK;  // no vectorised shape fits: keep the scalar kernels
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    bool found = false;
+    for (auto &t : g_tune)
+      if (t.nx == best.nx && t.ny == best.ny && t.P == best.P && t.sponge == best.sponge && t.cov == best.cov) {
+        t = best;
+        found = true;
+      }
+    if (!found) g_tune.push_back(best);
+  }
+  return finish(best);
 }
 
 int tomo_set_shape(tomo_ctx *c, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth) {

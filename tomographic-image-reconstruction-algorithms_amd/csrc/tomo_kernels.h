@@ -46,6 +46,8 @@ struct FwdArgs {
   T c0dt_h;             // c0 dt / h
   int sponge_w;
   const T *sp_a, *sp_ab;
+  const int2 *ring;     // cells of the edge/receiver pass {flat cell, receiver or -1}
+  int nring;
 };
 
 struct AdjArgs {
@@ -69,6 +71,19 @@ struct AdjArgs {
   const unsigned char *row_has_recv;  // per row: 1 if the row holds a receiver
   int sponge_w;
   const float *sp_a, *sp_ab;
+  const int2 *ring;     // cells of the edge/receiver pass
+  int nring;
+};
+
+// Kernel parameter of the stencil kernels: one problem (a), or a batch of
+// problems whose descriptors live in device memory, item p owning the
+// blocks [start[p], start[p + 1]) of the launch.
+template <typename Args>
+struct Launch {
+  Args a;
+  const Args *items;
+  const int *start;
+  int n;
 };
 
 // Stencil kernel shape: vec = cells per lane along x (1: scalar kernels,
@@ -94,6 +109,19 @@ template <typename T>
 hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, const StencilShape &sh, hipStream_t st);
 hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape &sh, hipStream_t st);
 
+// Batched stencil launches (vectorised kernels, sponge preset): the main
+// kernel over blocks_main blocks (item p: start_main[p] ..), then / before it
+// the edge/receiver kernel over blocks_ring blocks (item p: start_ring[p] ..,
+// shots x ceil(nring / 256) blocks).  Item descriptors carry tiles_x and
+// tiles_per_shot for the shape (vec, rows).
+hipError_t launch_forward_batch(const FwdArgs<float> *items, const int *start_main, const int *start_ring, int n,
+                                int blocks_main, int blocks_ring, int vec, int rows, int depth, hipStream_t st);
+hipError_t launch_adjoint_batch(const AdjArgs *items, const int *start_main, const int *start_ring, int n,
+                                int blocks_main, int blocks_ring, bool cov, int vec, int rows, int depth,
+                                hipStream_t st);
+// tiles of one shot for a vectorised shape
+void vec_tiles(int nx, int ny, int vec, int rows, int *tiles_x, int *tiles_per_shot);
+
 // pw = a(d) * (c0 dt / h)^2 * (1 + f)
 template <typename T>
 hipError_t launch_coefficients(const float *f, T *pw, int nx, int ny, double c0dt_h, int sponge_w,
@@ -110,17 +138,29 @@ struct Mask {
   int x0, x1, y0, y1;
 };
 
-// f += scale * mask * (sum_g G_g) / (1 + f) / div
+// f += scale * mask * (sum_g G_g) / (1 + f) / div  [* pixw per pixel when given]
 hipError_t launch_update_uniform(float *f, const float *G, int nshots, int nx, int ny, Mask m,
-                                 double scale, double div, hipStream_t st);
-// f += scale * mask * G_g / (1 + f)
+                                 double scale, double div, hipStream_t st, const float *pixw = nullptr);
+// f += scale * mask * G_g / (1 + f)  [* pixw]
 hipError_t launch_update_group(float *f, const float *Gg, int nx, int ny, Mask m, double scale,
-                               hipStream_t st);
+                               hipStream_t st, const float *pixw = nullptr);
+// per-pixel illumination weights w = cbar / (c + eps cmax) from the sum c of
+// S coverage images (stats: 2 doubles of scratch)
+hipError_t launch_pixel_weights(const float *cov, int S, int nx, int ny, Mask m, double eps, double *stats,
+                                float *w, hipStream_t st);
 // weighted SIRT: mean image (FP64), per-shot squared distance partials,
 // weights (normalised, degenerate fallback), update.
 hipError_t launch_weighted(float *f, const float *G, int M, int nx, int ny, Mask m, double scale,
                            double *mean_scratch, double *part_scratch, int part_blocks,
-                           double *w_out, hipStream_t st);
+                           double *w_out, hipStream_t st, const float *pixw = nullptr);
+// the weights of weighted SIRT only (no update)
+hipError_t launch_weights_only(const float *f, const float *G, int M, int nx, int ny, double *mean, double *part,
+                               int part_blocks, double *w, hipStream_t st);
+// d = update direction of unit relaxation (uniform when w is null, else
+// weighted), then out4 = {sum (f-fo)^2, sum (f-fo) d, sum d^2, sum fo^2}
+hipError_t launch_error_quadratic(const float *f, const float *fo, const float *G, int M, int nx, int ny, Mask m,
+                                  double div, const double *w, const float *pixw, double *d, double *out4,
+                                  hipStream_t st);
 // sums {sum (f - fo)^2, sum fo^2} in FP64 into out[2]
 hipError_t launch_error(const float *f, const float *fo, int n, double *out, hipStream_t st);
 

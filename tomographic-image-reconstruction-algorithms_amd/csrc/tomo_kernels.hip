@@ -306,6 +306,21 @@ __device__ __forceinline__ int xcd_logical_block() {
   return xcd * per + min(xcd, rem) + loc;
 }
 
+// the problem of this block in a batched launch; lb becomes the block index
+// within that problem
+template <bool BATCH, typename Args>
+__device__ __forceinline__ Args pick_item(const Launch<Args> &L, int &lb) {
+  if (!BATCH) return L.a;
+  int lo = 0, hi = L.n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (L.start[mid] <= lb) lo = mid;
+    else hi = mid;
+  }
+  lb -= L.start[lo];
+  return L.items[lo];
+}
+
 template <int VEC>
 struct Row {
   float e[VEC];
@@ -368,10 +383,11 @@ __device__ __forceinline__ void row_ends(const Row<VEC> &r, const float *__restr
 // record are done by forward_ring_kernel right after, so the hot loop has no
 // cell map.  Needs nx >= 4 VEC.
 
-template <bool SPONGE, int VEC, int RY, int PD>
-__global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<float> A) {
+template <bool SPONGE, int VEC, int RY, int PD, bool BATCH>
+__global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<FwdArgs<float>> L) {
+  int lb = xcd_logical_block();
+  const FwdArgs<float> A = pick_item<BATCH>(L, lb);
   const int nx = A.nx, ny = A.ny;
-  const int lb = xcd_logical_block();
   const int tile = lb % A.tiles_per_shot, shot = lb / A.tiles_per_shot;
   const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
   const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
@@ -475,14 +491,22 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(FwdArgs<f
 // Edge cells and receiver record of one forward step, for the vectorised
 // kernels (one thread per listed cell and shot; runs after forward_vec_kernel
 // on the same stream).  cells[i] = {flat cell, receiver index or -1}.
-template <bool SPONGE>
-__global__ __launch_bounds__(256) void forward_ring_kernel(FwdArgs<float> A, const int2 *__restrict__ cells,
-                                                           int ncells) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncells) return;
-  const int2 c = cells[i];
-  const int gs = A.shot0 + blockIdx.y;
-  const long long so = (long long)blockIdx.y * A.sstride;
+template <bool SPONGE, bool BATCH>
+__global__ __launch_bounds__(256) void forward_ring_kernel(Launch<FwdArgs<float>> L) {
+  int lb = blockIdx.x, shot, i;
+  const FwdArgs<float> A = pick_item<BATCH>(L, lb);
+  if (BATCH) {
+    const int chunks = (A.nring + 255) / 256;
+    shot = lb / chunks;
+    i = (lb % chunks) * 256 + threadIdx.x;
+  } else {
+    shot = blockIdx.y;
+    i = blockIdx.x * blockDim.x + threadIdx.x;
+  }
+  if (i >= A.nring) return;
+  const int2 c = A.ring[i];
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
   const int x = c.x % A.nx, y = c.x / A.nx;
   float val;
   if (c.y < 0) {
@@ -503,10 +527,11 @@ __global__ __launch_bounds__(256) void forward_ring_kernel(FwdArgs<float> A, con
 // their side-edge cells, receivers excepted (rows holding receivers are
 // flagged per row; their receiver cells are skipped): adjoint_ring_kernel
 // does the receivers and the top and bottom rows.
-template <bool SPONGE, bool COV, int VEC, int RY, int PD>
-__global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A) {
+template <bool SPONGE, bool COV, int VEC, int RY, int PD, bool BATCH>
+__global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<AdjArgs> L) {
+  int lb = xcd_logical_block();
+  const AdjArgs A = pick_item<BATCH>(L, lb);
   const int nx = A.nx, ny = A.ny;
-  const int lb = xcd_logical_block();
   const int tile = lb % A.tiles_per_shot, shot = lb / A.tiles_per_shot;
   const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
   const int y0 = ((tile / A.tiles_x) * kBlockY + threadIdx.y) * RY;
@@ -651,13 +676,22 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(AdjArgs A
 // Edge, corner and receiver cells of one adjoint step for the vectorised
 // kernels (the cells adjoint_vec_kernel skips); inputs are the previous
 // levels only, so the order against adjoint_vec_kernel does not matter.
-template <bool SPONGE, bool COV>
-__global__ __launch_bounds__(256) void adjoint_ring_kernel(AdjArgs A, const int2 *__restrict__ cells, int ncells) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncells) return;
-  const int2 c = cells[i];
-  const int gs = A.shot0 + blockIdx.y;
-  const long long so = (long long)blockIdx.y * A.sstride;
+template <bool SPONGE, bool COV, bool BATCH>
+__global__ __launch_bounds__(256) void adjoint_ring_kernel(Launch<AdjArgs> L) {
+  int lb = blockIdx.x, shot, i;
+  const AdjArgs A = pick_item<BATCH>(L, lb);
+  if (BATCH) {
+    const int chunks = (A.nring + 255) / 256;
+    shot = lb / chunks;
+    i = (lb % chunks) * 256 + threadIdx.x;
+  } else {
+    shot = blockIdx.y;
+    i = blockIdx.x * blockDim.x + threadIdx.x;
+  }
+  if (i >= A.nring) return;
+  const int2 c = A.ring[i];
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
   const long long n2 = (long long)A.nx * A.ny;
   const int x = c.x % A.nx, y = c.x / A.nx;
   float zval, lu, uval;
@@ -721,23 +755,25 @@ __device__ __forceinline__ bool in_mask(const Mask &m, int x, int y) {
 
 // uniform SIRT (sirt-mimo.cu:858-877; sirt-mimo-io.cu:774-796)
 __global__ void update_uniform_kernel(float *__restrict__ f, const float *__restrict__ G, int S, int nx,
-                                      int ny, Mask m, double scale, double div) {
+                                      int ny, Mask m, double scale, double div, const float *__restrict__ pixw) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   const long long n2 = (long long)nx * ny;
   if (id >= n2 || !in_mask(m, id % nx, id / nx)) return;
   double s = 0.0;
   for (int g = 0; g < S; ++g) s += (double)G[g * n2 + id];
   const double fv = f[id];
-  f[id] = (float)(fv + scale * (s / (1.0 + fv)) / div);
+  const double d = scale * (s / (1.0 + fv)) / div;
+  f[id] = (float)(fv + (pixw ? d * (double)pixw[id] : d));
 }
 
 // ART step (art-mimo-io.cu:381-400)
 __global__ void update_group_kernel(float *__restrict__ f, const float *__restrict__ Gg, int nx, int ny,
-                                    Mask m, double scale) {
+                                    Mask m, double scale, const float *__restrict__ pixw) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nx * ny || !in_mask(m, id % nx, id / nx)) return;
   const double fv = f[id];
-  f[id] = (float)(fv + scale * ((double)Gg[id] / (1.0 + fv)));
+  const double d = scale * ((double)Gg[id] / (1.0 + fv));
+  f[id] = (float)(fv + (pixw ? d * (double)pixw[id] : d));
 }
 
 // weighted SIRT, pass 1: mean increment image (weights_differential1's
@@ -801,14 +837,102 @@ __global__ __launch_bounds__(256) void weighted_weights_kernel(const double *__r
 // pass 4: f += scale * mask * sum_m w_m df_m (average_differential + update_field,
 // sirt-mimo-norm.cu:100-118, 873-892)
 __global__ void weighted_update_kernel(float *__restrict__ f, const float *__restrict__ G, int M, int nx,
-                                       int ny, Mask msk, double scale, const double *__restrict__ w) {
+                                       int ny, Mask msk, double scale, const double *__restrict__ w,
+                                       const float *__restrict__ pixw) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   const long long n2 = (long long)nx * ny;
   if (id >= n2 || !in_mask(msk, id % nx, id / nx)) return;
   const double q = 1.0 + (double)f[id];
   double s = 0.0;
   for (int m = 0; m < M; ++m) s += w[m] * ((double)G[m * n2 + id] / q);
-  f[id] = (float)((double)f[id] + scale * s);
+  const double d = scale * s;
+  f[id] = (float)((double)f[id] + (pixw ? d * (double)pixw[id] : d));
+}
+
+// Update direction of one unit of relaxation, d = mask * update(s = 1), in
+// FP64: uniform (w == nullptr: sum_g G_g / (1 + f) / div) or weighted
+// (sum_m w_m G_m / (1 + f)); pixw optional
+__global__ void direction_kernel(const float *__restrict__ f, const float *__restrict__ G, int M, int nx, int ny,
+                                 Mask msk, double div, const double *__restrict__ w, const float *__restrict__ pixw,
+                                 double *__restrict__ d) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n2 = (long long)nx * ny;
+  if (id >= n2) return;
+  if (!in_mask(msk, id % nx, id / nx)) {
+    d[id] = 0.0;
+    return;
+  }
+  const double q = 1.0 + (double)f[id];
+  double s = 0.0;
+  if (w) {
+    for (int m = 0; m < M; ++m) s += w[m] * ((double)G[m * n2 + id] / q);
+  } else {
+    for (int m = 0; m < M; ++m) s += (double)G[m * n2 + id];
+    s = s / q / div;
+  }
+  d[id] = pixw ? s * (double)pixw[id] : s;
+}
+
+// out[0..3] = {sum (f - fo)^2, sum (f - fo) d, sum d^2, sum fo^2}
+__global__ __launch_bounds__(256) void quadratic_kernel(const float *__restrict__ f, const float *__restrict__ fo,
+                                                        const double *__restrict__ d, long long n,
+                                                        double *__restrict__ out) {
+  __shared__ double sh[256];
+  double a = 0.0, b = 0.0, c = 0.0, e = 0.0;
+  for (long long i = threadIdx.x; i < n; i += 256) {
+    const double r = (double)f[i] - (double)fo[i];
+    a += r * r;
+    b += r * d[i];
+    c += d[i] * d[i];
+    e += (double)fo[i] * (double)fo[i];
+  }
+  const double sa = block_sum_256(a, sh);
+  const double sb = block_sum_256(b, sh);
+  const double sc = block_sum_256(c, sh);
+  const double se = block_sum_256(e, sh);
+  if (threadIdx.x == 0) {
+    out[0] = sa;
+    out[1] = sb;
+    out[2] = sc;
+    out[3] = se;
+  }
+}
+
+// Illumination (coverage) normalisation: stats = {mean of c over the mask,
+// max of c}, c = sum of the S coverage images
+__global__ __launch_bounds__(256) void coverage_stats_kernel(const float *__restrict__ cov, int S, int nx, int ny,
+                                                             Mask m, double *__restrict__ stats) {
+  __shared__ double sh[256];
+  const long long n2 = (long long)nx * ny;
+  double sum = 0.0, mx = 0.0;
+  for (long long i = threadIdx.x; i < n2; i += 256) {
+    double c = 0.0;
+    for (int g = 0; g < S; ++g) c += (double)cov[g * n2 + i];
+    if (in_mask(m, (int)(i % nx), (int)(i / nx))) sum += c;
+    mx = fmax(mx, c);
+  }
+  const double ssum = block_sum_256(sum, sh);
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    stats[0] = ssum / fmax((double)(m.x1 - m.x0) * (double)(m.y1 - m.y0), 1.0);
+    stats[1] = sh[0];
+  }
+}
+
+// w(x) = cbar / (c(x) + eps cmax)
+__global__ void pixel_weight_kernel(const float *__restrict__ cov, int S, long long n2, double eps,
+                                    const double *__restrict__ stats, float *__restrict__ w) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  double c = 0.0;
+  for (int g = 0; g < S; ++g) c += (double)cov[g * n2 + i];
+  const double den = c + eps * stats[1];
+  w[i] = den > 0.0 ? (float)(stats[0] / den) : 0.f;
 }
 
 __global__ __launch_bounds__(256) void error_kernel(const float *__restrict__ f, const float *__restrict__ fo,
@@ -854,28 +978,36 @@ inline dim3 vec_grid(int nx, int ny, int vec, int ry, int nshots, int *tiles_x, 
   return dim3(*tiles_per_shot * nshots, 1, 1);
 }
 
-template <bool SPONGE, int VEC, int PD>
-void fwd_vec_rows(FwdArgs<float> a, dim3 grid, dim3 block, int ry, hipStream_t st) {
+template <bool SPONGE, int VEC, int PD, bool BATCH>
+void fwd_vec_rows(const Launch<FwdArgs<float>> &L, dim3 grid, dim3 block, int ry, hipStream_t st) {
   switch (ry) {
-    case 4: forward_vec_kernel<SPONGE, VEC, 4, PD><<<grid, block, 0, st>>>(a); break;
-    case 8: forward_vec_kernel<SPONGE, VEC, 8, PD><<<grid, block, 0, st>>>(a); break;
-    case 32: forward_vec_kernel<SPONGE, VEC, 32, PD><<<grid, block, 0, st>>>(a); break;
-    default: forward_vec_kernel<SPONGE, VEC, 16, PD><<<grid, block, 0, st>>>(a); break;
+    case 4: forward_vec_kernel<SPONGE, VEC, 4, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+    case 8: forward_vec_kernel<SPONGE, VEC, 8, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+    case 32: forward_vec_kernel<SPONGE, VEC, 32, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+    default: forward_vec_kernel<SPONGE, VEC, 16, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+  }
+}
+
+template <bool SPONGE, int VEC, bool BATCH>
+void fwd_vec_depth(const Launch<FwdArgs<float>> &L, dim3 grid, int ry, int pd, hipStream_t st) {
+  const dim3 block(kBlockX, kBlockY, 1);
+  switch (pd) {
+    case 1: fwd_vec_rows<SPONGE, VEC, 1, BATCH>(L, grid, block, ry, st); break;
+    case 4: fwd_vec_rows<SPONGE, VEC, 4, BATCH>(L, grid, block, ry, st); break;
+    default: fwd_vec_rows<SPONGE, VEC, 2, BATCH>(L, grid, block, ry, st); break;
   }
 }
 
 template <bool SPONGE, int VEC>
 hipError_t fwd_vec_dispatch(FwdArgs<float> a, int nshots, int ry, int pd, const int2 *ring, int nring,
                             hipStream_t st) {
-  const dim3 block(kBlockX, kBlockY, 1);
   const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x, &a.tiles_per_shot);
-  switch (pd) {
-    case 1: fwd_vec_rows<SPONGE, VEC, 1>(a, grid, block, ry, st); break;
-    case 4: fwd_vec_rows<SPONGE, VEC, 4>(a, grid, block, ry, st); break;
-    default: fwd_vec_rows<SPONGE, VEC, 2>(a, grid, block, ry, st); break;
-  }
+  a.ring = ring;
+  a.nring = nring;
+  const Launch<FwdArgs<float>> L{a, nullptr, nullptr, 0};
+  fwd_vec_depth<SPONGE, VEC, false>(L, grid, ry, pd, st);
   // edge cells and receiver record read this step's interior result: after
-  if (nring > 0) forward_ring_kernel<SPONGE><<<dim3((nring + 255) / 256, nshots, 1), 256, 0, st>>>(a, ring, nring);
+  if (nring > 0) forward_ring_kernel<SPONGE, false><<<dim3((nring + 255) / 256, nshots, 1), 256, 0, st>>>(L);
   return hipGetLastError();
 }
 
@@ -890,26 +1022,35 @@ hipError_t adj_dispatch(AdjArgs a, int nshots, int ry, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <bool SPONGE, bool COV, int VEC, int PD>
-void adj_vec_rows(AdjArgs a, dim3 grid, dim3 block, int ry, hipStream_t st) {
+template <bool SPONGE, bool COV, int VEC, int PD, bool BATCH>
+void adj_vec_rows(const Launch<AdjArgs> &L, dim3 grid, dim3 block, int ry, hipStream_t st) {
   switch (ry) {
-    case 4: adjoint_vec_kernel<SPONGE, COV, VEC, 4, PD><<<grid, block, 0, st>>>(a); break;
-    case 8: adjoint_vec_kernel<SPONGE, COV, VEC, 8, PD><<<grid, block, 0, st>>>(a); break;
-    case 32: adjoint_vec_kernel<SPONGE, COV, VEC, 32, PD><<<grid, block, 0, st>>>(a); break;
-    default: adjoint_vec_kernel<SPONGE, COV, VEC, 16, PD><<<grid, block, 0, st>>>(a); break;
+    case 4: adjoint_vec_kernel<SPONGE, COV, VEC, 4, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+    case 8: adjoint_vec_kernel<SPONGE, COV, VEC, 8, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+    case 32: adjoint_vec_kernel<SPONGE, COV, VEC, 32, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+    default: adjoint_vec_kernel<SPONGE, COV, VEC, 16, PD, BATCH><<<grid, block, 0, st>>>(L); break;
+  }
+}
+
+template <bool SPONGE, bool COV, int VEC, bool BATCH>
+void adj_vec_depth(const Launch<AdjArgs> &L, dim3 grid, int ry, int pd, hipStream_t st) {
+  const dim3 block(kBlockX, kBlockY, 1);
+  switch (pd) {
+    case 1: adj_vec_rows<SPONGE, COV, VEC, 1, BATCH>(L, grid, block, ry, st); break;
+    case 4: adj_vec_rows<SPONGE, COV, VEC, 4, BATCH>(L, grid, block, ry, st); break;
+    default: adj_vec_rows<SPONGE, COV, VEC, 2, BATCH>(L, grid, block, ry, st); break;
   }
 }
 
 template <bool SPONGE, bool COV, int VEC>
 hipError_t adj_vec_dispatch(AdjArgs a, int nshots, int ry, int pd, const int2 *ring, int nring, hipStream_t st) {
-  const dim3 block(kBlockX, kBlockY, 1);
   const dim3 grid = vec_grid(a.nx, a.ny, VEC, ry, nshots, &a.tiles_x, &a.tiles_per_shot);
-  if (nring > 0) adjoint_ring_kernel<SPONGE, COV><<<dim3((nring + 255) / 256, nshots, 1), 256, 0, st>>>(a, ring, nring);
-  switch (pd) {
-    case 1: adj_vec_rows<SPONGE, COV, VEC, 1>(a, grid, block, ry, st); break;
-    case 4: adj_vec_rows<SPONGE, COV, VEC, 4>(a, grid, block, ry, st); break;
-    default: adj_vec_rows<SPONGE, COV, VEC, 2>(a, grid, block, ry, st); break;
-  }
+  a.ring = ring;
+  a.nring = nring;
+  const Launch<AdjArgs> L{a, nullptr, nullptr, 0};
+  if (nring > 0)
+    adjoint_ring_kernel<SPONGE, COV, false><<<dim3((nring + 255) / 256, nshots, 1), 256, 0, st>>>(L);
+  adj_vec_depth<SPONGE, COV, VEC, false>(L, grid, ry, pd, st);
   return hipGetLastError();
 }
 
@@ -999,35 +1140,109 @@ hipError_t launch_misfit(const float *rr, long long per_shot, int nshots, double
 }
 
 hipError_t launch_update_uniform(float *f, const float *G, int nshots, int nx, int ny, Mask m, double scale,
-                                 double div, hipStream_t st) {
+                                 double div, hipStream_t st, const float *pixw) {
   const int n = nx * ny;
-  update_uniform_kernel<<<(n + 255) / 256, 256, 0, st>>>(f, G, nshots, nx, ny, m, scale, div);
+  update_uniform_kernel<<<(n + 255) / 256, 256, 0, st>>>(f, G, nshots, nx, ny, m, scale, div, pixw);
   return hipGetLastError();
 }
 
 hipError_t launch_update_group(float *f, const float *Gg, int nx, int ny, Mask m, double scale,
-                               hipStream_t st) {
+                               hipStream_t st, const float *pixw) {
   const int n = nx * ny;
-  update_group_kernel<<<(n + 255) / 256, 256, 0, st>>>(f, Gg, nx, ny, m, scale);
+  update_group_kernel<<<(n + 255) / 256, 256, 0, st>>>(f, Gg, nx, ny, m, scale, pixw);
+  return hipGetLastError();
+}
+
+hipError_t launch_pixel_weights(const float *cov, int S, int nx, int ny, Mask m, double eps, double *stats,
+                                float *w, hipStream_t st) {
+  const long long n2 = (long long)nx * ny;
+  coverage_stats_kernel<<<1, 256, 0, st>>>(cov, S, nx, ny, m, stats);
+  pixel_weight_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, st>>>(cov, S, n2, eps, stats, w);
   return hipGetLastError();
 }
 
 int weighted_part_blocks(int n2) { return std::max(1, std::min(256, (n2 + 4095) / 4096)); }
 
 hipError_t launch_weighted(float *f, const float *G, int M, int nx, int ny, Mask m, double scale,
-                           double *mean, double *part, int part_blocks, double *w, hipStream_t st) {
+                           double *mean, double *part, int part_blocks, double *w, hipStream_t st,
+                           const float *pixw) {
   const long long n2 = (long long)nx * ny;
   const int nb = (int)((n2 + 255) / 256);
   weighted_mean_kernel<<<nb, 256, 0, st>>>(f, G, M, n2, mean);
   const int chunk = (int)((n2 + part_blocks - 1) / part_blocks);
   weighted_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, n2, chunk, part);
   weighted_weights_kernel<<<1, 256, 0, st>>>(part, M, part_blocks, w);
-  weighted_update_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, w);
+  weighted_update_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, w, pixw);
   return hipGetLastError();
 }
 
 hipError_t launch_error(const float *f, const float *fo, int n, double *out, hipStream_t st) {
   error_kernel<<<1, 256, 0, st>>>(f, fo, n, out);
+  return hipGetLastError();
+}
+
+void vec_tiles(int nx, int ny, int vec, int rows, int *tiles_x, int *tiles_per_shot) {
+  (void)vec_grid(nx, ny, vec, rows, 1, tiles_x, tiles_per_shot);
+}
+
+static int batch_rows(int rows) { return (rows == 4 || rows == 8 || rows == 32) ? rows : 16; }
+
+hipError_t launch_forward_batch(const FwdArgs<float> *items, const int *start_main, const int *start_ring, int n,
+                                int blocks_main, int blocks_ring, int vec, int rows, int depth, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int ry = batch_rows(rows);
+  const Launch<FwdArgs<float>> Lm{FwdArgs<float>{}, items, start_main, n};
+  if (vec == 4) fwd_vec_depth<true, 4, true>(Lm, dim3(blocks_main, 1, 1), ry, depth, st);
+  else if (vec == 2) fwd_vec_depth<true, 2, true>(Lm, dim3(blocks_main, 1, 1), ry, depth, st);
+  else return hipErrorInvalidValue;
+  if (blocks_ring > 0) {
+    const Launch<FwdArgs<float>> Lr{FwdArgs<float>{}, items, start_ring, n};
+    forward_ring_kernel<true, true><<<dim3(blocks_ring, 1, 1), 256, 0, st>>>(Lr);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_batch(const AdjArgs *items, const int *start_main, const int *start_ring, int n,
+                                int blocks_main, int blocks_ring, bool cov, int vec, int rows, int depth,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int ry = batch_rows(rows);
+  const Launch<AdjArgs> Lr{AdjArgs{}, items, start_ring, n};
+  const Launch<AdjArgs> Lm{AdjArgs{}, items, start_main, n};
+  if (blocks_ring > 0) {
+    if (cov) adjoint_ring_kernel<true, true, true><<<dim3(blocks_ring, 1, 1), 256, 0, st>>>(Lr);
+    else adjoint_ring_kernel<true, false, true><<<dim3(blocks_ring, 1, 1), 256, 0, st>>>(Lr);
+  }
+  const dim3 grid(blocks_main, 1, 1);
+  if (vec == 4) {
+    if (cov) adj_vec_depth<true, true, 4, true>(Lm, grid, ry, depth, st);
+    else adj_vec_depth<true, false, 4, true>(Lm, grid, ry, depth, st);
+  } else if (vec == 2) {
+    if (cov) adj_vec_depth<true, true, 2, true>(Lm, grid, ry, depth, st);
+    else adj_vec_depth<true, false, 2, true>(Lm, grid, ry, depth, st);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_error_quadratic(const float *f, const float *fo, const float *G, int M, int nx, int ny, Mask m,
+                                  double div, const double *w, const float *pixw, double *d, double *out4,
+                                  hipStream_t st) {
+  const long long n2 = (long long)nx * ny;
+  direction_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, st>>>(f, G, M, nx, ny, m, div, w, pixw, d);
+  quadratic_kernel<<<1, 256, 0, st>>>(f, fo, d, n2, out4);
+  return hipGetLastError();
+}
+
+hipError_t launch_weights_only(const float *f, const float *G, int M, int nx, int ny, double *mean, double *part,
+                               int part_blocks, double *w, hipStream_t st) {
+  const long long n2 = (long long)nx * ny;
+  const int nb = (int)((n2 + 255) / 256);
+  weighted_mean_kernel<<<nb, 256, 0, st>>>(f, G, M, n2, mean);
+  const int chunk = (int)((n2 + part_blocks - 1) / part_blocks);
+  weighted_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, n2, chunk, part);
+  weighted_weights_kernel<<<1, 256, 0, st>>>(part, M, part_blocks, w);
   return hipGetLastError();
 }
 

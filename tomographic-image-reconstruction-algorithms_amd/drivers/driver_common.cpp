@@ -95,8 +95,9 @@ int run_loop(tomo_ctx *ctx, const Loop &L, double (*scale_fn)(int, const void *)
     const float scale = (float)scale_fn(iter, user);
     double misfit = 0, eps = 0;
     int rc;
-    if (L.rule == Rule::Art) rc = tomo_art_iteration(ctx, scale, &misfit, &eps);
-    else rc = tomo_sirt_iteration(ctx, scale, L.rule == Rule::Weighted, L.mode, &misfit, &eps);
+    const bool art = L.rule == Rule::Art || (L.rule == Rule::ArtThenWeighted && iter < L.art_iterations);
+    if (art) rc = tomo_art_iteration(ctx, scale, &misfit, &eps);
+    else rc = tomo_sirt_iteration(ctx, scale, L.rule != Rule::Uniform, L.mode, &misfit, &eps);
     check(rc, ctx, "iteration");
     curr = (float)eps;
     const float t = std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
@@ -114,6 +115,12 @@ int run_loop(tomo_ctx *ctx, const Loop &L, double (*scale_fn)(int, const void *)
     if (curr <= L.target_epsilon) {
       std::cerr << "reached target epsilon = " << L.target_epsilon << ", at iter = " << shown
                 << ", epsilon = " << curr << "\n\n";
+      // the ART warm start only ends its own phase (sirt-mimo-seq.cu:410-414, 508)
+      if (L.rule == Rule::ArtThenWeighted && iter < L.art_iterations) {
+        prev = curr;
+        iter = L.art_iterations - 1;
+        continue;
+      }
       break;
     }
     if (curr > prev || std::isnan(curr)) {

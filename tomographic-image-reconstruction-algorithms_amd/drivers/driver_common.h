@@ -24,7 +24,9 @@ tomo_ctx *reference_context(int group_size, int shots_per_pass, int *ng);
 // the reference's message) when a file is missing.
 bool load_inputs(tomo_ctx *ctx, const std::string &fo_filename, int group_size, std::vector<float> &fo);
 
-enum class Rule { Uniform, Weighted, Art };
+// Art then Weighted: the first art_iterations iterations are ART, the rest
+// weighted SIRT (sirt-mimo-seq.cu:337-596)
+enum class Rule { Uniform, Weighted, Art, ArtThenWeighted };
 
 struct Loop {
   Rule rule;
@@ -35,6 +37,7 @@ struct Loop {
   std::string convergence_file, time_file;
   bool table = true;           // the io drivers' iter/epsilon/difference table
   bool one_based = false;      // iteration numbers printed from 1 (norm-mem drivers)
+  int art_iterations = 5;      // ArtThenWeighted: length of the ART phase
 };
 
 // Run the loop; scale(k, user) gives the relaxation of 0-based iteration k.

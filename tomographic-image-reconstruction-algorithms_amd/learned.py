@@ -183,14 +183,27 @@ def run_learned(net: Preconditioner, p: Problem, scale: float, iters: int, alpha
     return tr
 
 
-def collect_plain(net: Preconditioner, p: Problem, scale: float, iters: int) -> None:
-    """Samples of a plain SIRT trajectory: at each iterate the field, the
-    plain step and the coverage (input) and truth - field (target)."""
+def collect_plain(net: Preconditioner, p: Problem, scale: float, iters: int, target: str = "best") -> float:
+    """Samples of a plain SIRT trajectory of `iters` iterations: at each
+    iterate the field, the plain step and the coverage (input).  Target:
+    "truth" - the true field minus the iterate; "best" - the trajectory's
+    iterate of lowest error (over all iterates, the last included) minus the
+    iterate, i.e. where plain SIRT itself gets to.  Returns that best error."""
     e = p.engine
+    first = net.num_samples()
+    best_eps, best_f = None, None
     for _ in range(iters):
         e.gradient()
         net.add_sample(e, scale)
         e.update_uniform(scale, UPDATE_MEAN)
+        eps = e.error()
+        if best_eps is None or eps < best_eps:
+            best_eps, best_f = eps, e.get_field()
+    if target == "best":
+        net.set_targets(first, iters, best_f)
+    elif target != "truth":
+        raise ValueError(target)
+    return float(best_eps)
 
 
 def collect_learned(net: Preconditioner, p: Problem, scale: float, iters: int, alpha: float = 1.0,
@@ -217,11 +230,12 @@ class TrainReport:
 
 def train(net: Preconditioner, n: int, seeds, iters: int, steps: int = 0, shots: int = 0, frac: float = 0.25,
           rounds: int = 1, train_steps: int = 600, batch: int = 8, lr: float = 1e-3, kind: str = "smooth",
-          alpha: float = 1.0, log=None) -> TrainReport:
-    """Train on plain-SIRT trajectories of phantoms `seeds`, then `rounds`
-    more rounds that add the states of the learned iteration on the same
-    phantoms and continue training.  One problem is alive at a time (its
-    data are regenerated per round).  Returns losses and timings."""
+          alpha: float = 1.0, target: str = "best", log=None) -> TrainReport:
+    """Train on plain-SIRT trajectories of phantoms `seeds` (targets as in
+    collect_plain), then `rounds` more rounds that add the states of the
+    learned iteration on the same phantoms (target: the truth) and continue
+    training.  One problem is alive at a time (its data are regenerated per
+    round).  Returns losses and timings."""
     rep = TrainReport()
     t_data = t_train = 0.0
     scales = {}
@@ -230,20 +244,24 @@ def train(net: Preconditioner, n: int, seeds, iters: int, steps: int = 0, shots:
         p = Problem(n, phantom(n, int(s), kind), steps=steps, shots=shots)
         scales[s] = calibrate_scale(p, frac)
         p.reset()
-        collect_plain(net, p, scales[s], iters)
+        best = collect_plain(net, p, scales[s], iters, target=target)
         p.close()
         if log:
-            log(f"plain trajectory of phantom {s}: {net.num_samples()} samples")
+            log(f"plain trajectory of phantom {s}: best eps {best:.2f}, {net.num_samples()} samples")
     t_data += time.perf_counter() - t
     t = time.perf_counter()
     rep.losses += list(net.train(train_steps, batch, lr, seed=0))
     t_train += time.perf_counter() - t
+    if log:
+        log(f"trained on plain trajectories: loss {rep.losses[0]:.4f} -> {np.mean(rep.losses[-50:]):.4f}")
     for r in range(rounds):
         t = time.perf_counter()
         for s in seeds:
             p = Problem(n, phantom(n, int(s), kind), steps=steps, shots=shots)
             collect_learned(net, p, scales[s], iters, alpha=alpha)
             p.close()
+            if log:
+                log(f"round {r + 1}: learned trajectory of phantom {s}: {net.num_samples()} samples")
         t_data += time.perf_counter() - t
         t = time.perf_counter()
         rep.losses += list(net.train(train_steps, batch, lr, seed=r + 1))
