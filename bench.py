@@ -13,9 +13,11 @@ checkpointed every 200 steps, smooth random velocity (Gaussian-filtered noise,
 sigma 8 cells, +-8 %) plus 4 sharp inclusions; measured traces from the FP64
 solver.
 
-Also reported: the roofline of the forward stencil kernel (algorithmic 12 B
+Also reported: the roofline of the forward stencil kernel (algorithmic 16 B
 per cell-update, average launch time from HIP events on the engine's
-stream), the adjoint kernel's figures, and the CPU baseline (the oracle's FP32
+stream; next to it the bytes this engine's layout moves, and the measured
+HBM bytes per launch from the committed counter summary), the adjoint
+kernel's figures (20 B), and the CPU baseline (the oracle's FP32
 transcription, OpenMP over shots) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -37,11 +39,25 @@ from _pkg import tomo  # noqa: E402
 import tomo_amd.workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per cell-update of one shot.  The per-cell
-# coefficient is one image shared by every shot of a launch, so it adds
-# 4/shots bytes (cached) and is not counted.
-FWD_BYTES = 12.0   # read u(k), u(k-1); write u(k+1)
-ADJ_BYTES = 24.0   # read z(k+1), z(k+2), u(k), gradient; write z(k), gradient
+# Algorithmic HBM bytes per cell-update of one shot, the per-unit figures of
+# the roofline: a forward cell-update reads u(k), u(k-1) and the per-cell
+# coefficient and writes u(k+1) (16 B); an adjoint cell-update fused with the
+# correlation reads z(k+1), z(k+2), the coefficient and the recomputed u(k)
+# and writes z(k), with the gradient accumulated in registers (20 B).
+FWD_BYTES = 16.0
+ADJ_BYTES = 20.0
+# What this engine's kernels move per cell-update by their own data layout:
+# the coefficient image is shared by every shot of a launch (4/shots B,
+# L2-resident); the two-step forward kernel reads u(k), u(k-1) and writes
+# u(k+1), u(k+2) once per two steps (8 B), the one-step kernel 12 B; the
+# adjoint kernel re-reads and re-writes the per-shot gradient each step
+# (24 B).
+FWD_LAYOUT_BYTES = {1: 12.0, 2: 8.0}
+ADJ_LAYOUT_BYTES = 24.0
+# Measured HBM bytes per launch of the stencil kernels (rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE of a --counter-pass run, scaled by the counters'
+# reading for a 1 GiB device copy).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stencil_hbm_traffic_c3_512x512x128.md")
 
 WORKLOADS = {
     # name: (n, steps, shots, checkpoint)
@@ -70,6 +86,63 @@ def cpu_baseline(c, ii, jj, fo, meas, threads, shots):
     return time.perf_counter() - t
 
 
+def load_traffic(kernel, shape, tb):
+    """HBM bytes per launch of `kernel` at this stencil shape and time
+    blocking, from the committed counter summary (None when it has no row
+    for them).  Rows read `| kernel | shape | steps per launch | ... |
+    bytes per launch | ...`."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            lines = fh.read().splitlines()
+    except OSError:
+        return None
+    want_shape = ",".join(str(v) for v in shape)
+    for ln in lines:
+        cols = [v.strip().strip("`") for v in ln.strip().strip("|").split("|")]
+        if len(cols) < 7 or not cols[0].startswith(kernel + "<"):
+            continue
+        if cols[1] == want_shape and cols[2] == str(tb):
+            try:
+                return float(cols[6])
+            except ValueError:
+                return None
+    return None
+
+
+def counter_pass(args):
+    """One FP32 forward (data) and one gradient of the C3 workload, then a
+    1 GiB device copy: the program of the rocprofv3 --pmc runs that measure
+    the stencil kernels' HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE; the
+    copy's reading gives the counters' scale)."""
+    import torch
+    log = (lambda m: print("[counters]", m, file=sys.stderr, flush=True))
+    c, ii, jj, fo = make_workload("c3")
+    e = tomo.Engine(c, device=0)
+    if args.shape:
+        e.set_shape(*[int(v) for v in args.shape.split(",")])
+    e.set_time_blocking(args.time_blocking)
+    e.set_transducers(ii, jj, one_based=False)
+    e.set_receivers(ii, jj, one_based=False)
+    e.set_groups(1)
+    e.set_field(fo)
+    log("forward")
+    meas = e.forward(fp64=False)
+    e.set_field(np.zeros(fo.size, np.float32))
+    e.set_measured(meas)
+    log("gradient")
+    e.gradient()
+    shape = list(e.get_shape())
+    tb = e.time_blocking()
+    e.close()
+    log("copy 1 GiB")
+    a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"counter_pass": "c3", "stencil_shape": shape, "forward_steps_per_launch": tb,
+                      "copy_bytes": 2 * a.numel() * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +150,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
     ap.add_argument("--graphs", action="store_true", help="replay each gradient pass as a captured HIP graph")
+    ap.add_argument("--study", choices=["c2", "ttt", "train"], default=None,
+                    help="run a convergence study instead of the throughput benchmark (one JSON line)")
+    ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
+    ap.add_argument("--out", default="gpurun_out/preconditioner.npy", help="parameters written by --study train")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
+    ap.add_argument("--time-blocking", type=int, default=2, choices=[1, 2],
+                    help="forward steps per launch (2: two steps with the intermediate level in registers)")
     ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows,fwd_depth,adj_depth (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--data", default="fp64", choices=["fp64", "fp32"],
@@ -86,6 +165,8 @@ def main():
                     help="after the run, copy 1 GiB device to device (a plain copy of known size, used to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="run one C3 forward and gradient plus a 1 GiB copy (the program for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -93,6 +174,23 @@ def main():
               "that path is not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.counter_pass:
+        counter_pass(args)
+        return
+    if args.study:
+        import tomo_amd.studies as S
+        log = (lambda m: print("[study]", m, file=sys.stderr, flush=True))
+        if args.study == "c2":
+            out = S.c2_study(log=log)
+        elif args.study == "train":
+            out = S.train_study(args.out, log=log)
+        else:
+            runs = [S.ttt_study(seed=int(v), log=log) for v in args.seeds.split(",")]
+            out = {"study": "ttt", "runs": runs,
+                   "speedups": [r["speedup"] for r in runs],
+                   "all_reached": all(r["learned_seconds"] is not None for r in runs)}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "c4":
         run_c4(args)
         return
@@ -103,6 +201,7 @@ def main():
         e.set_shape(*[int(v) for v in args.shape.split(",")])
     if args.graphs:
         e.set_graphs(True)
+    e.set_time_blocking(args.time_blocking)
     e.set_transducers(ii, jj, one_based=False)
     e.set_receivers(ii, jj, one_based=False)
     e.set_groups(1)
@@ -138,11 +237,21 @@ def main():
     cells = tm["fwd_cells"] + tm["adj_cells"] + e.timing_graphs()["cells"]
     value = cells / elapsed / 1e9
 
-    cells_per_launch = shots * n * n
-    fwd_avg_ms = tm["fwd_ms"] / max(tm["fwd_launches"], 1)
+    # a forward launch advances every shot of the pass by `tb` levels; the
+    # engine's spans count levels and cover each launch with its edge passes
+    tb = e.time_blocking()
+    shape = tuple(e.get_shape())
+    fwd_cells_launch = shots * n * n * tb
+    adj_cells_launch = shots * n * n
+    fwd_avg_ms = tm["fwd_ms"] / max(tm["fwd_launches"], 1) * tb
     adj_avg_ms = tm["adj_ms"] / max(tm["adj_launches"], 1)
-    fwd_gbs = FWD_BYTES * cells_per_launch / (fwd_avg_ms * 1e-3) / 1e9
-    adj_gbs = ADJ_BYTES * cells_per_launch / (adj_avg_ms * 1e-3) / 1e9
+    fwd_gbs = FWD_BYTES * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
+    adj_gbs = ADJ_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
+    fwd_layout_gbs = FWD_LAYOUT_BYTES[tb] * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
+    adj_layout_gbs = ADJ_LAYOUT_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
+    fwd_kernel = "forward2_kernel" if tb == 2 else "forward_vec_kernel"
+    fwd_traffic = load_traffic(fwd_kernel, shape, tb) if args.workload == "c3" else None
+    adj_traffic = load_traffic("adjoint_vec_kernel", shape, tb) if args.workload == "c3" else None
 
     out = {
         "metric": "Gcell-updates/s of FDTD step",
@@ -161,20 +270,26 @@ def main():
             "workload": f"batched plain SIRT iteration: {n}x{n} grid, {shots} ring shots, {steps} steps, "
                         f"checkpoint every {ckpt or 'none'}",
             "grid": n, "shots": shots, "time_steps": steps, "checkpoint_every": ckpt,
-            "cell_updates_per_step": cells / args.steps, "stencil_shape": list(e.get_shape()),
+            "cell_updates_per_step": cells / args.steps, "stencil_shape": list(shape),
+            "forward_steps_per_launch": tb,
             "eps_after_steps": eps[-1] if eps else None,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "forward_vec_kernel",
+            "bound": "hbm", "kernel": f"{fwd_kernel} (with its edge/receiver passes)",
             "achieved": round(fwd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_cell_update": FWD_BYTES, "cells_per_launch": cells_per_launch,
+            "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": fwd_traffic,
+            "bytes_per_cell_update": FWD_BYTES, "cells_per_launch": fwd_cells_launch,
             "avg_launch_ms": round(fwd_avg_ms, 5),
+            "layout_bytes_per_cell_update": FWD_LAYOUT_BYTES[tb], "layout_achieved": round(fwd_layout_gbs, 1),
+            "layout_frac": round(fwd_layout_gbs / HBM_PEAK_GBS, 4),
         },
         "adjoint": {
-            "kernel": "adjoint_vec_kernel", "achieved_GBs": round(adj_gbs, 1),
-            "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "bytes_per_cell_update": ADJ_BYTES,
+            "kernel": "adjoint_vec_kernel (with its edge/receiver pass)", "achieved": round(adj_gbs, 1),
+            "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "traffic": adj_traffic,
+            "bytes_per_cell_update": ADJ_BYTES, "cells_per_launch": adj_cells_launch,
             "avg_launch_ms": round(adj_avg_ms, 5),
+            "layout_bytes_per_cell_update": ADJ_LAYOUT_BYTES, "layout_achieved": round(adj_layout_gbs, 1),
+            "layout_frac": round(adj_layout_gbs / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": None,
     }
@@ -241,8 +356,9 @@ def run_c4(args):
     eps = float(np.mean([e.error() for e in engines]))
     per_call_ms = tb["ms"] / max(tb["calls"], 1)
     cells_call = tb["cells"] / max(tb["calls"], 1)
-    # algorithmic bytes of a gradient call: 12 B per forward and recompute
-    # cell-update, 24 B per adjoint cell-update (one third of the updates)
+    # algorithmic bytes of a gradient call: FWD_BYTES per forward and
+    # recompute cell-update, ADJ_BYTES per adjoint cell-update (one third of
+    # the updates)
     gbs = (cells_call * (2 * FWD_BYTES + ADJ_BYTES) / 3) / (per_call_ms * 1e-3) / 1e9
     out = {
         "metric": "Gcell-updates/s of FDTD step",

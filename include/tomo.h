@@ -326,6 +326,15 @@ int tomo_timing_reset(tomo_ctx *ctx);
  * sirt-mimo.cu:165-299). */
 int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth);
 
+/* Forward steps per launch in the gradient passes: 2 (default) runs two
+ * leapfrog steps in one pass over each strip (the intermediate level kept in
+ * registers: 8 instead of 12 bytes of traffic per cell-update) where it
+ * applies -- the sponge preset, vectorised kernels, nx of 128, 256 or 512,
+ * checkpoint interval >= 4 -- and 1 otherwise; results are the same.
+ * tomo_get_time_blocking returns the steps per launch in effect. */
+int tomo_set_time_blocking(tomo_ctx *ctx, int steps);
+int tomo_get_time_blocking(const tomo_ctx *ctx);
+
 /* Autotuner: times the vectorised forward and adjoint kernels of this
  * context (its grid, shots per pass, sponge and coverage) over the shapes
  * vec in {4, 2} x rows in {8, 16, 32} x prefetch depth in {1, 2, 4}

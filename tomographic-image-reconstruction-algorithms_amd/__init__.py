@@ -125,6 +125,8 @@ SIGNATURES = {
     "tomo_set_shape": (_int, [_vp, _int, _int, _int, _int, _int]),
     "tomo_get_shape": (_int, [_vp, _i32p]),
     "tomo_set_graphs": (_int, [_vp, _int]),
+    "tomo_set_time_blocking": (_int, [_vp, _int]),
+    "tomo_get_time_blocking": (_int, [_vp]),
     "tomo_autotune": (_int, [_vp, _int, _i32p, _f64p]),
     "tomo_timing_graphs": (_int, [_vp, _f64p]),
     # batch of problems
@@ -444,6 +446,13 @@ class Engine:
         out = np.zeros(3, np.float64)
         self._c(lib().tomo_timing_graphs(self._ctx, out))
         return {"ms": out[0], "launches": int(out[1]), "cells": out[2]}
+
+    def set_time_blocking(self, steps: int):
+        """Forward steps per launch in the gradient passes (1 or 2)."""
+        self._c(lib().tomo_set_time_blocking(self._ctx, int(steps)))
+
+    def time_blocking(self) -> int:
+        return self._c(lib().tomo_get_time_blocking(self._ctx))
 
     def autotune(self, force=False):
         """Time the stencil shapes on this context and keep the fastest; returns (shape, (fwd_ms, adj_ms))."""

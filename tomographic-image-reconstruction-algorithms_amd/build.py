@@ -25,8 +25,10 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 (v_pk_*_f32)
-# instructions in device code.
-HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# instructions in device code.  -ffp-contract=off: no automatic FMA
+# contraction; the kernels write their fused multiply-adds explicitly, so the
+# rounding of a cell does not depend on how a kernel variant was scheduled.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]

@@ -282,7 +282,17 @@ int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
   return TOMO_OK;
 }
 
-// location of forward level k during the first (recording) pass
+// two forward steps per launch in the gradient passes: the sponge preset,
+// the vectorised kernels, full-row waves, and room for a 4-level ring
+bool two_step(const tomo_ctx *c) {
+  if (c->time_block < 2 || !forward2_supported(c->nx, c->cfg.sponge_width)) return false;
+  if (stencil_shape(c).vec < 2) return false;
+  return full_history(c) || c->cfg.checkpoint_every >= 4;
+}
+
+// location of forward level k during the first (recording) pass (the
+// rolling levels use a ring of 4 when two steps share a launch: level k+2 is
+// written while other strips still read level k-1)
 float *pass1_level(tomo_ctx *c, int P, int k, bool need_history) {
   const size_t lvl = (size_t)P * c->n2;
   if (!need_history) return c->d_seg + (size_t)(k % 3) * lvl;
@@ -290,7 +300,7 @@ float *pass1_level(tomo_ctx *c, int P, int k, bool need_history) {
   const int C = c->cfg.checkpoint_every;
   const int q = k % C;
   if (q < 2) return c->d_ckpt + ((size_t)(k / C) * 2 + q) * lvl;
-  return c->d_seg + (size_t)(k % 3) * lvl;
+  return c->d_seg + (size_t)(k % (two_step(c) ? 4 : 3)) * lvl;
 }
 
 // location of level k while segment m is being replayed
@@ -351,6 +361,63 @@ int forward_step(tomo_ctx *c, FwdArgs<T> a, const T *um, const T *u0, T *up, int
   if (sh.vec == 1 && sh.fwd_rows < 4) sh.fwd_rows = 4;
   sh.rows = sh.fwd_rows;
   HIPCHK(c, launch_forward_step<T>(a, P, sh, c->stream));
+  return TOMO_OK;
+}
+
+// Forward steps k and k+1 (levels k-1, k -> k+1, k+2) in one main launch,
+// with the edge passes: top/bottom rows of k+1 before; receiver records of
+// k+1, top/bottom rows and receiver records of k+2 after.
+int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u0, float *up1, float *up2, int k,
+                  int P, bool record, const float *meas, float *rec) {
+  const bool rec_on = record && rec != nullptr && c->nr > 0;
+  a.meas = meas;
+  a.rec = rec;
+  a.ring = c->d_ring;
+  // rows 0 and ny-1 of level k+1
+  a.um = um;
+  a.u0 = u0;
+  a.up = up1;
+  a.level = k + 1;
+  a.record = 0;
+  a.nring = 2 * c->nx;
+  HIPCHK(c, launch_forward_ring(a, P, c->stream));
+  Fwd2Args b{};
+  b.um = um;
+  b.u0 = u0;
+  b.up1 = up1;
+  b.up2 = up2;
+  b.sstride = a.sstride;
+  b.pw = a.pw;
+  b.f = a.f;
+  b.bbox = a.bbox;
+  b.nx = c->nx;
+  b.ny = c->ny;
+  b.shot0 = a.shot0;
+  b.amp1 = pulse_h2<float>(c, k);
+  b.amp2 = pulse_h2<float>(c, k + 1);
+  b.c0dt_h = a.c0dt_h;
+  b.sponge_w = a.sponge_w;
+  b.sp_a = a.sp_a;
+  b.sp_ab = a.sp_ab;
+  HIPCHK(c, launch_forward2(b, P, stencil_shape(c).fwd_rows, c->stream));
+  // receiver records of level k+1
+  if (rec_on && k + 1 >= c->cfg.record_first) {
+    a.up = up1;
+    a.level = k + 1;
+    a.record = 1;
+    a.ring = c->d_ring + 2 * c->nx;
+    a.nring = c->nr;
+    HIPCHK(c, launch_forward_ring(a, P, c->stream));
+  }
+  // rows 0 and ny-1 and receiver records of level k+2
+  a.um = u0;
+  a.u0 = up1;
+  a.up = up2;
+  a.level = k + 2;
+  a.record = rec_on && (k + 2 >= c->cfg.record_first);
+  a.ring = c->d_ring;
+  a.nring = 2 * c->nx + c->nr;
+  HIPCHK(c, launch_forward_ring(a, P, c->stream));
   return TOMO_OK;
 }
 
@@ -422,9 +489,19 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   {
     TimedSpan s;
     span_begin(c, s, 0);
-    for (int k = 1; k < nt - 1; ++k) {
-      int rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
-                                   pass1_level(c, P, k + 1, true), k, np, true, c->d_meas, c->d_rr);
+    const bool two = two_step(c);
+    for (int k = 1; k < nt - 1;) {
+      int rc;
+      if (two && k + 1 < nt - 1) {
+        rc = forward_step2(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
+                           pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true), k, np, true, c->d_meas,
+                           c->d_rr);
+        k += 2;
+      } else {
+        rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
+                                 pass1_level(c, P, k + 1, true), k, np, true, c->d_meas, c->d_rr);
+        k += 1;
+      }
       if (rc) return rc;
     }
     s.launches = nt - 2;
@@ -451,10 +528,20 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
     if (!full && cnt > 2) {
       TimedSpan s;
       span_begin(c, s, 0);
-      for (int q = 2; q < cnt; ++q) {
+      const bool two = two_step(c);
+      for (int q = 2; q < cnt;) {
         const int k = k0 + q - 1;
-        int rc = forward_step<float>(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
-                                     segment_level(c, P, m, k + 1), k, np, false, nullptr, nullptr);
+        int rc;
+        if (two && q + 1 < cnt) {
+          rc = forward_step2(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
+                             segment_level(c, P, m, k + 1), segment_level(c, P, m, k + 2), k, np, false, nullptr,
+                             nullptr);
+          q += 2;
+        } else {
+          rc = forward_step<float>(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
+                                   segment_level(c, P, m, k + 1), k, np, false, nullptr, nullptr);
+          q += 1;
+        }
         if (rc) return rc;
       }
       s.launches = cnt - 2;
@@ -492,7 +579,7 @@ std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
                          (uintptr_t)c->d_rowflag, (uintptr_t)c->d_spa, (uintptr_t)c->d_spab,
                          (uintptr_t)c->coverage, (uintptr_t)c->nshots, (uintptr_t)c->nr, (uintptr_t)sh.vec,
                          (uintptr_t)sh.fwd_rows, (uintptr_t)sh.adj_rows, (uintptr_t)sh.fwd_depth,
-                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every};
+                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c)};
   return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
 }
 
@@ -1155,6 +1242,15 @@ int tomo_line_search(tomo_ctx *c, int weighted, double s_lo, double s_step, int 
   if (best_eps) *best_eps = 100.0 * std::sqrt(std::max(err2(kbest), 0.0)) / std::sqrt(q[3]);
   return TOMO_OK;
 }
+
+int tomo_set_time_blocking(tomo_ctx *c, int steps) {
+  if (!c) return TOMO_ERR_ARG;
+  if (steps != 1 && steps != 2) return fail(c, TOMO_ERR_ARG, "steps per launch must be 1 or 2");
+  c->time_block = steps;
+  return TOMO_OK;
+}
+
+int tomo_get_time_blocking(const tomo_ctx *c) { return c ? (two_step(c) ? 2 : 1) : TOMO_ERR_ARG; }
 
 int tomo_set_pixel_weights(tomo_ctx *c, int mode, double eps) {
   int rc = check_ctx(c);

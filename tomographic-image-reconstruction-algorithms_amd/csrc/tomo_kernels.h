@@ -109,6 +109,31 @@ template <typename T>
 hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, const StencilShape &sh, hipStream_t st);
 hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape &sh, hipStream_t st);
 
+// Two forward steps k -> k+1 -> k+2 in one launch (sponge preset, waves of
+// full rows: nx == 64 vec, vec in {2, 4, 8}).  Rows 0 and ny-1 of level k+1
+// must be in up1 before (launch_forward_ring with the top/bottom cells); rows
+// 0 and ny-1 of level k+2 and the receiver records are left to the edge pass
+// after.
+struct Fwd2Args {
+  const float *um, *u0;  // levels k-1, k of local shot 0
+  float *up1, *up2;      // levels k+1, k+2
+  long long sstride;
+  const float *pw;
+  const float *f;
+  const int4 *bbox;      // per global shot
+  int nx, ny;
+  int shot0;
+  int blocks_per_shot;
+  float amp1, amp2;      // pulse * h^2 of steps k and k+1 (0: no source)
+  float c0dt_h;
+  int sponge_w;
+  const float *sp_a, *sp_ab;
+};
+bool forward2_supported(int nx, int sponge_w);
+hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st);
+// the edge/receiver pass alone over a.ring[0 .. a.nring) (grid: cells x shots)
+hipError_t launch_forward_ring(const FwdArgs<float> &a, int nshots, hipStream_t st);
+
 // Batched stencil launches (vectorised kernels, sponge preset): the main
 // kernel over blocks_main blocks (item p: start_main[p] ..), then / before it
 // the edge/receiver kernel over blocks_ring blocks (item p: start_ring[p] ..,

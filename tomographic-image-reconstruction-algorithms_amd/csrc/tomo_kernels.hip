@@ -21,16 +21,52 @@ __device__ __forceinline__ int edge_dist(int x, int y, int nx, int ny) {
   return min(min(x, y), min(nx - 1 - x, ny - 1 - y));
 }
 
+// ------------------------------------------------------- cell arithmetic
+// Device code is compiled without automatic FMA contraction; every cell rule
+// below is one fixed chain of fused multiply-adds, shared by all kernels that
+// apply it (scalar, vectorised, two-step, ring, batched), so each of them
+// rounds a cell exactly as the others do and their fields agree bit for bit.
+
+__device__ __forceinline__ float fused(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fused(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Leapfrog interior update (propagation, sirt-mimo.cu:540-563, written with
+// the per-cell weight pw = c0^2 dt^2 (1 + f) / h^2 and the sponge factors a,
+// ab): u(k+1) = pw * sum4 + (2 a - 4 pw) u(k) - ab u(k-1), where sum4 is the
+// sum of the four neighbours of u(k)
+template <typename T>
+__device__ __forceinline__ T leapfrog(T pw, T sum4, T u, T a, T ab, T m) {
+  return fused(-ab, m, fused(fused(T(-4), pw, T(2) * a), u, pw * sum4));
+}
+
+// Adjoint interior update (backpropagation1, sirt-mimo.cu:686-710): the
+// weighted neighbour sum sum_n pw_n z_n + (2 a - 4 pw) z(k+1) - ab z(k+2)
+__device__ __forceinline__ float adjoint_rule(float pn, float zn, float ps, float zs, float pl, float zl, float pr,
+                                              float zr, float pc, float zc, float a, float ab, float z2) {
+  const float s = fused(pr, zr, fused(pl, zl, fused(ps, zs, pn * zn)));
+  return fused(-ab, z2, fused(fused(-4.f, pc, 2.f * a), zc, s));
+}
+
+// 5-point Laplacian (laplace, sirt-mimo.cu:776-804)
+__device__ __forceinline__ float lap5(float un, float us, float ul, float ur, float uc, float inv_h2) {
+  return fused(-4.f, uc, ((un + us) + ul) + ur) * inv_h2;
+}
+
 // ----------------------------------------------------------------- forward
 
 // One-way edge rule of propagation (sirt-mimo.cu:566-587): corners use the
 // diagonal neighbour.
 // u(k+1) at an edge cell from u(k) at the cell (u), one (ua) and two (ub)
-// cells inward, and u(k-1) at the cell (m) and one cell inward (ma)
+// cells inward, and u(k-1) at the cell (m) and one cell inward (ma):
+// (2 - 2r - r^2) u + 2r (1 + r) ua - r^2 ub + (2r - 1) m - 2r ma
 template <typename T>
 __device__ __forceinline__ T mur_edge(T r, T u, T ua, T ub, T m, T ma) {
-  return (T(2) - T(2) * r - r * r) * u + T(2) * r * (T(1) + r) * ua - r * r * ub + (T(2) * r - T(1)) * m -
-         T(2) * r * ma;
+  const T r2 = T(2) * r, rr = r * r;
+  T v = (T(2) - r2 - rr) * u;
+  v = fused(r2 * (T(1) + r), ua, v);
+  v = fused(-rr, ub, v);
+  v = fused(r2 - T(1), m, v);
+  return fused(-r2, ma, v);
 }
 
 template <typename T>
@@ -98,18 +134,16 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<
     T val;
     if (xin && y > 0 && y < ny - 1) {
       const T sum = ((r[i] + l[i]) + c[i]) + c[i + 2];  // u(i+1) + u(i-1) + u(j-1) + u(j+1)
+      T a = T(1), ab = T(1);
       if (SPONGE) {
         const int d = edge_dist(x, y, nx, ny);
-        T a = T(1), ab = T(1);
         if (d < A.sponge_w) {
           a = A.sp_a[d];
           ab = A.sp_ab[d];
         }
-        val = p[i] * sum + (T(2) * a - T(4) * p[i]) * c[i + 1] - ab * m[i];
-      } else {
-        val = p[i] * sum + (T(2) - T(4) * p[i]) * c[i + 1] - m[i];
       }
-      if (src_col && y >= bb.z && y <= bb.w) val += p[i] * A.amp_h2;
+      val = leapfrog<T>(p[i], sum, c[i + 1], a, ab, m[i]);
+      if (src_col && y >= bb.z && y <= bb.w) val = fused(p[i], T(A.amp_h2), val);
     } else {
       val = forward_edge(A, u0, um, x, y);
     }
@@ -146,9 +180,8 @@ __device__ float adjoint_interior(const AdjArgs &A, const float *__restrict__ z1
   const float *__restrict__ pw = A.pw;
   float a, ab;
   sponge_factors<SPONGE>(A, x, y, a, ab);
-  return (((pw[id - nx] * z1[id - nx] + pw[id + nx] * z1[id + nx]) + pw[id - 1] * z1[id - 1]) +
-          pw[id + 1] * z1[id + 1]) +
-         (2.f * a - 4.f * pw[id]) * z1[id] - ab * z2[id];
+  return adjoint_rule(pw[id - nx], z1[id - nx], pw[id + nx], z1[id + nx], pw[id - 1], z1[id - 1], pw[id + 1],
+                      z1[id + 1], pw[id], z1[id], a, ab, z2[id]);
 }
 
 // forward level with the corner cells replaced by the mean of their two edge
@@ -166,10 +199,8 @@ __device__ __forceinline__ float u_corner_mean(const float *__restrict__ u, int 
 __device__ float lap_clamped(const float *__restrict__ u, int nx, int ny, int x, int y, float inv_h2) {
   const int ya = y > 0 ? y - 1 : y, yb = y < ny - 1 ? y + 1 : y;
   const int xa = x > 0 ? x - 1 : x, xb = x < nx - 1 ? x + 1 : x;
-  return (((u_corner_mean(u, nx, ny, x, ya) + u_corner_mean(u, nx, ny, x, yb)) +
-           u_corner_mean(u, nx, ny, xa, y)) +
-          u_corner_mean(u, nx, ny, xb, y) - 4.f * u_corner_mean(u, nx, ny, x, y)) *
-         inv_h2;
+  return lap5(u_corner_mean(u, nx, ny, x, ya), u_corner_mean(u, nx, ny, x, yb), u_corner_mean(u, nx, ny, xa, y),
+              u_corner_mean(u, nx, ny, xb, y), u_corner_mean(u, nx, ny, x, y), inv_h2);
 }
 
 // Slow path: edge, corner and receiver cells.  Final z(k) follows
@@ -189,8 +220,8 @@ __device__ void adjoint_slow(const AdjArgs &A, const float *__restrict__ z1, con
     zval = adjoint_interior<SPONGE>(A, z1, z2, xi, yi);
   } else {
     const int an = A.anchor[ri];
-    zval = adjoint_interior<SPONGE>(A, z1, z2, an % nx, an / nx) +
-           A.rr[((long long)gs * A.nt + A.level) * A.nr + ri] * A.gain;
+    zval = fused(A.rr[((long long)gs * A.nt + A.level) * A.nr + ri], A.gain,
+                 adjoint_interior<SPONGE>(A, z1, z2, an % nx, an / nx));
   }
   if (ex && ey) {
     lu = (lap_clamped(u, nx, ny, xi, y, A.inv_h2) + lap_clamped(u, nx, ny, x, yi, A.inv_h2)) / 2.f;
@@ -264,16 +295,16 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_step_kernel(AdjArgs 
     if (interior && cm[i] < 0) {
       float a, ab;
       sponge_factors<SPONGE>(A, x, y, a, ab);
-      zval = (((pc[i] * zc[i] + pc[i + 2] * zc[i + 2]) + pl[i] * zl[i]) + pr[i] * zr[i]) +
-             (2.f * a - 4.f * pc[i + 1]) * zc[i + 1] - ab * zz[i];
-      lu = (((uc[i] + uc[i + 2]) + ul[i]) + ur[i] - 4.f * uc[i + 1]) * A.inv_h2;
+      zval = adjoint_rule(pc[i], zc[i], pc[i + 2], zc[i + 2], pl[i], zl[i], pr[i], zr[i], pc[i + 1], zc[i + 1], a,
+                          ab, zz[i]);
+      lu = lap5(uc[i], uc[i + 2], ul[i], ur[i], uc[i + 1], A.inv_h2);
       uval = uc[i + 1];
     } else {
       adjoint_slow<SPONGE>(A, z1, z2, u, gs, x, y, interior ? cm[i] : -1, zval, lu, uval);
     }
     z0[id] = zval;
-    acc[id] = ac[i] + zval * lu;
-    if (COV) cov[id] = cv[i] + uval * uval;
+    acc[id] = fused(zval, lu, ac[i]);
+    if (COV) cov[id] = fused(uval, uval, cv[i]);
   }
 }
 
@@ -321,15 +352,62 @@ __device__ __forceinline__ Args pick_item(const Launch<Args> &L, int &lb) {
   return L.items[lo];
 }
 
+// A row segment of VEC cells; hl / hr hold the cells just left / right of
+// the wave's segment for the tile-edge lanes (load_row_halo).
 template <int VEC>
 struct Row {
   float e[VEC];
+  float hl, hr;
 };
+
+// Sponge factors of a lane's VEC cells.  The factor depends on the cell's
+// distance to the nearest edge, min(side distance, row distance): the side
+// part is fixed per lane, so its factors sit in registers for the whole
+// strip, and only rows within the sponge width of the top or bottom edge
+// (one branch per row, the same for the whole wave) look the table up.
+template <int VEC>
+struct LaneSponge {
+  float a[VEC], ab[VEC];
+  int dx[VEC];
+};
+
+template <int VEC>
+__device__ __forceinline__ LaneSponge<VEC> lane_sponge(int x0, int nx, int w, const float *__restrict__ sp_a,
+                                                       const float *__restrict__ sp_ab, bool ok) {
+  LaneSponge<VEC> s;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    const int x = x0 + e;
+    s.dx[e] = min(x, nx - 1 - x);
+    s.a[e] = 1.f;
+    s.ab[e] = 1.f;
+    if (ok && s.dx[e] < w) {
+      s.a[e] = sp_a[s.dx[e]];
+      s.ab[e] = sp_ab[s.dx[e]];
+    }
+  }
+  return s;
+}
+
+template <int VEC>
+__device__ __forceinline__ void sponge_at(const LaneSponge<VEC> &s, int e, int y, int ny, int w, const float *sp_a,
+                                          const float *sp_ab, float &a, float &ab) {
+  const int dy = min(y, ny - 1 - y);
+  if (dy >= w) {
+    a = s.a[e];
+    ab = s.ab[e];
+  } else {
+    const int d = min(s.dx[e], dy);
+    a = sp_a[d];
+    ab = sp_ab[d];
+  }
+}
 
 template <int VEC>
 __device__ __forceinline__ Row<VEC> load_row(const float *__restrict__ p, bool ok) {
   typedef float vt __attribute__((ext_vector_type(VEC)));
   Row<VEC> r;
+  r.hl = r.hr = 0.f;
   if (ok) {
     const vt v = *reinterpret_cast<const vt *>(p);
 #pragma unroll
@@ -340,6 +418,38 @@ __device__ __forceinline__ Row<VEC> load_row(const float *__restrict__ p, bool o
   }
   return r;
 }
+
+// Row y (lane segment at x0) with the tile-edge lanes' outer neighbours, all
+// issued together so that no load is waited for when the row is consumed.
+template <int VEC>
+__device__ __forceinline__ Row<VEC> load_row_halo(const float *__restrict__ base, int y, int x0, int nx, bool ok) {
+  Row<VEC> r = load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
+  if (ok && threadIdx.x == 0 && x0 > 0) r.hl = base[y * nx + x0 - 1];
+  if (ok && threadIdx.x == kBlockX - 1 && x0 + VEC < nx) r.hr = base[y * nx + x0 + VEC];
+  return r;
+}
+
+// left / right neighbours of a lane's segment: DPP within the wave, the
+// prefetched halo at the tile edges (0 outside the grid)
+template <int VEC>
+__device__ __forceinline__ void halo_ends(const Row<VEC> &r, float &lft, float &rgt) {
+  const float l = from_prev_lane(r.e[VEC - 1]);
+  const float g = from_next_lane(r.e[0]);
+  lft = threadIdx.x == 0 ? r.hl : l;
+  rgt = threadIdx.x == kBlockX - 1 ? r.hr : g;
+}
+
+// f at the side-edge cells of row y for the lanes holding them (else 0)
+template <int VEC>
+__device__ __forceinline__ float edge_f(const float *__restrict__ f, int y, int x0, int nx, bool ok) {
+  if (!ok) return 0.f;
+  if (x0 == 0) return f[y * nx];
+  if (x0 + VEC == nx) return f[y * nx + nx - 1];
+  return 0.f;
+}
+
+// Sponge tables in LDS (filled by the whole block before any wave returns)
+constexpr int kSpongeLds = 256;
 
 template <int VEC>
 __device__ __forceinline__ void store_row(float *__restrict__ p, const Row<VEC> &r) {
@@ -387,6 +497,14 @@ template <bool SPONGE, int VEC, int RY, int PD, bool BATCH>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<FwdArgs<float>> L) {
   int lb = xcd_logical_block();
   const FwdArgs<float> A = pick_item<BATCH>(L, lb);
+  __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
+  if (SPONGE) {
+    for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
+      s_spa[i] = A.sp_a[i];
+      s_spab[i] = A.sp_ab[i];
+    }
+    __syncthreads();
+  }
   const int nx = A.nx, ny = A.ny;
   const int tile = lb % A.tiles_per_shot, shot = lb / A.tiles_per_shot;
   const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
@@ -402,40 +520,46 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
   const int4 bb = A.bbox[gs];
   const bool src = A.amp_h2 != 0.f;
   const int yend = min(y0 + RY, ny);  // rows of this strip: [y0, yend)
+  const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, SPONGE ? A.sponge_w : 0, s_spa, s_spab, xok);
 
   // u(k) rows y0-1 .. yend are needed; u(k-1) and the coefficients rows y0 .. yend-1
   auto uok = [&](int y) { return xok && y >= y0 - 1 && y <= yend && y >= 0 && y < ny; };
   auto sok = [&](int y) { return xok && y >= y0 && y < yend; };
   auto at = [&](int y, bool ok) { return ok ? y * nx + x0 : 0; };
 
-  // queue of the next PD rows: qu[j] = u(k) row y+1+j, qm/qp = u(k-1) / pw row y+j
+  // queue of the next PD rows: qu[j] = u(k) row y+1+j (with the tile-edge
+  // halo), qm/qp = u(k-1) / pw row y+j, qf = f at the row's side edges
   Row<VEC> cu = load_row<VEC>(u0 + at(y0 - 1, uok(y0 - 1)), uok(y0 - 1));
-  Row<VEC> cc = load_row<VEC>(u0 + at(y0, uok(y0)), uok(y0));
+  Row<VEC> cc = load_row_halo<VEC>(u0, y0, x0, nx, uok(y0));
   Row<VEC> qu[PD], qm[PD], qp[PD];
+  float qf[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) {
-    qu[j] = load_row<VEC>(u0 + at(y0 + 1 + j, uok(y0 + 1 + j)), uok(y0 + 1 + j));
+    qu[j] = load_row_halo<VEC>(u0, y0 + 1 + j, x0, nx, uok(y0 + 1 + j));
     qm[j] = load_row<VEC>(um + at(y0 + j, sok(y0 + j)), sok(y0 + j));
     qp[j] = load_row<VEC>(pw + at(y0 + j, sok(y0 + j)), sok(y0 + j));
+    qf[j] = edge_f<VEC>(A.f, y0 + j, x0, nx, sok(y0 + j));
   }
 
 #pragma unroll 1
   for (int base = y0; base < yend; base += PD) {
     // loads of the next PD rows are in flight while these PD rows compute
     Row<VEC> nu[PD], nm[PD], np[PD];
+    float nf[PD];
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int yu = base + PD + 1 + j, ys = base + PD + j;
-      nu[j] = load_row<VEC>(u0 + at(yu, uok(yu)), uok(yu));
+      nu[j] = load_row_halo<VEC>(u0, yu, x0, nx, uok(yu));
       nm[j] = load_row<VEC>(um + at(ys, sok(ys)), sok(ys));
       np[j] = load_row<VEC>(pw + at(ys, sok(ys)), sok(ys));
+      nf[j] = edge_f<VEC>(A.f, ys, x0, nx, sok(ys));
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int y = base + j;
       if (y < yend) {  // wavefront-uniform
         float lft, rgt;
-        row_ends<VEC>(cc, u0 + (xok ? y * nx : 0), x0, nx, xok, lft, rgt);
+        halo_ends<VEC>(cc, lft, rgt);
         if (xok && y > 0 && y < ny - 1) {
           Row<VEC> out;
           const bool src_row = src && y >= bb.z && y <= bb.w;
@@ -446,30 +570,21 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
             const float r = e == VEC - 1 ? rgt : cc.e[e + 1];
             const float sum = ((r + l) + cu.e[e]) + qu[j].e[e];
             const float pe = qp[j].e[e];
-            float val;
-            if (SPONGE) {
-              const int d = edge_dist(x, y, nx, ny);
-              float a = 1.f, ab = 1.f;
-              if (d < A.sponge_w) {
-                a = A.sp_a[d];
-                ab = A.sp_ab[d];
-              }
-              val = pe * sum + (2.f * a - 4.f * pe) * cc.e[e] - ab * qm[j].e[e];
-            } else {
-              val = pe * sum + (2.f - 4.f * pe) * cc.e[e] - qm[j].e[e];
-            }
-            if (src_row && x >= bb.x && x <= bb.y) val += pe * A.amp_h2;
+            float a = 1.f, ab = 1.f;
+            if (SPONGE) sponge_at<VEC>(ls, e, y, ny, A.sponge_w, s_spa, s_spab, a, ab);
+            float val = leapfrog<float>(pe, sum, cc.e[e], a, ab, qm[j].e[e]);
+            if (src_row && x >= bb.x && x <= bb.y) val = fused(pe, A.amp_h2, val);
             out.e[e] = val;
           }
           // side edges of this row: every operand is in this lane's registers
           if (x0 == 0) {
-            const float r = A.c0dt_h * sqrtf(1.f + A.f[y * nx]);
+            const float r = A.c0dt_h * sqrtf(1.f + qf[j]);
             out.e[0] = mur_edge<float>(r, cc.e[0], cc.e[1], VEC == 2 ? rgt : cc.e[2 % VEC], qm[j].e[0],
                                        qm[j].e[1]);
           }
           if (x0 + VEC == nx) {
             constexpr int E = VEC - 1;
-            const float r = A.c0dt_h * sqrtf(1.f + A.f[y * nx + nx - 1]);
+            const float r = A.c0dt_h * sqrtf(1.f + qf[j]);
             out.e[E] = mur_edge<float>(r, cc.e[E], cc.e[E - 1], VEC == 2 ? lft : cc.e[(E + VEC - 2) % VEC],
                                        qm[j].e[E], qm[j].e[E - 1]);
           }
@@ -484,7 +599,122 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
       qu[j] = nu[j];
       qm[j] = nm[j];
       qp[j] = np[j];
+      qf[j] = nf[j];
     }
+  }
+}
+
+// Two leapfrog steps k -> k+1 -> k+2 in one pass over a strip of RY rows
+// (sponge preset; a wave holds full rows, nx == 64 VEC, so x-neighbours move
+// by DPP only).  Each lane walks rows r = y0-1 .. yend: it forms u(k+1) at
+// row r from the u(k) window (rows r-1, r, r+1), stores it on its own rows,
+// and forms u(k+2) at row r-1 from the u(k+1) window (rows r-2, r-1, r) and
+// u(k) row r-1.  u(k) and u(k-1) are read once per strip (plus two halo rows
+// of u(k)), u(k+1) is kept in registers: 8 B of traffic per cell-update
+// instead of 12.  Rows 0 and ny-1 of u(k+1) come from the edge pass before
+// (they are read back); those of u(k+2) are left to the edge pass after.
+// The per-cell arithmetic is that of forward_vec_kernel.
+template <int VEC>
+__device__ __forceinline__ Row<VEC> step_row(const Fwd2Args &A, const LaneSponge<VEC> &ls, const float *s_spa,
+                                             const float *s_spab, const Row<VEC> &up, const Row<VEC> &at,
+                                             const Row<VEC> &dn, const Row<VEC> &prev, const Row<VEC> &p,
+                                             float fedge, int y, int x0, float amp, const int4 &bb) {
+  const int nx = A.nx, ny = A.ny;
+  const float lft = from_prev_lane(at.e[VEC - 1]);
+  const float rgt = from_next_lane(at.e[0]);
+  const bool src_row = amp != 0.f && y >= bb.z && y <= bb.w;
+  Row<VEC> out;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    const int x = x0 + e;
+    const float l = e == 0 ? lft : at.e[e - 1];
+    const float r = e == VEC - 1 ? rgt : at.e[e + 1];
+    const float sum = ((r + l) + up.e[e]) + dn.e[e];
+    const float pe = p.e[e];
+    float a, ab;
+    sponge_at<VEC>(ls, e, y, ny, A.sponge_w, s_spa, s_spab, a, ab);
+    float val = leapfrog<float>(pe, sum, at.e[e], a, ab, prev.e[e]);
+    if (src_row && x >= bb.x && x <= bb.y) val = fused(pe, amp, val);
+    out.e[e] = val;
+  }
+  if (x0 == 0) {
+    const float r = A.c0dt_h * sqrtf(1.f + fedge);
+    out.e[0] = mur_edge<float>(r, at.e[0], at.e[1], VEC == 2 ? rgt : at.e[2 % VEC], prev.e[0], prev.e[1]);
+  }
+  if (x0 + VEC == nx) {
+    constexpr int E = VEC - 1;
+    const float r = A.c0dt_h * sqrtf(1.f + fedge);
+    out.e[E] = mur_edge<float>(r, at.e[E], at.e[E - 1], VEC == 2 ? lft : at.e[(E + VEC - 2) % VEC], prev.e[E],
+                               prev.e[E - 1]);
+  }
+  return out;
+}
+
+template <int VEC, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) {
+  const int nx = A.nx, ny = A.ny;
+  __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
+  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
+    s_spa[i] = A.sp_a[i];
+    s_spab[i] = A.sp_ab[i];
+  }
+  __syncthreads();
+  const int lb = xcd_logical_block();
+  const int shot = lb / A.blocks_per_shot;
+  const int y0 = ((lb % A.blocks_per_shot) * kBlockY + threadIdx.y) * RY;
+  if (y0 >= ny) return;  // whole wavefront
+  const int yend = min(y0 + RY, ny);
+  const int x0 = threadIdx.x * VEC;
+  const long long so = (long long)shot * A.sstride;
+  const float *__restrict__ u0 = A.u0 + so;
+  const float *__restrict__ um = A.um + so;
+  float *__restrict__ up1 = A.up1 + so;
+  float *__restrict__ up2 = A.up2 + so;
+  const float *__restrict__ pw = A.pw;
+  const int4 bb = A.bbox[A.shot0 + shot];
+  const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
+  auto ld = [&](const float *__restrict__ base, int y) {
+    const bool ok = y >= 0 && y < ny;
+    return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
+  };
+  const int r0 = max(y0 - 1, 0), r1 = min(yend, ny - 1);
+  Row<VEC> cu = ld(u0, r0 - 1), cc = ld(u0, r0), cn = ld(u0, r0 + 1);
+  Row<VEC> mc = ld(um, r0), pc = ld(pw, r0);
+  float fc = edge_f<VEC>(A.f, r0, x0, nx, true);
+  // the edge rows of u(k+1) (written by the edge pass before) are read ahead
+  Row<VEC> e1 = ld(up1, r0);
+  Row<VEC> pp = pc, v2 = cc, v1 = cc;
+  float fp = fc;
+#pragma unroll 1
+  for (int r = r0; r <= r1; ++r) {
+    // loads of the next row are in flight while this row computes
+    const Row<VEC> nn = ld(u0, r + 2), nm = ld(um, r + 1), np = ld(pw, r + 1);
+    const float nf = edge_f<VEC>(A.f, r + 1, x0, nx, r + 1 < ny);
+    const bool edge_next = r + 1 == ny - 1;
+    const Row<VEC> ne = ld(up1, edge_next ? r + 1 : -1);
+    Row<VEC> v0;
+    if (r == 0 || r == ny - 1) {
+      v0 = e1;  // edge row, from the edge pass
+    } else {
+      v0 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, mc, pc, fc, r, x0, A.amp1, bb);
+      if (r >= y0 && r < yend) store_row<VEC>(up1 + r * nx + x0, v0);
+    }
+    const int q = r - 1;
+    if (q >= y0 && q >= 1 && q <= ny - 2) {
+      const Row<VEC> w = step_row<VEC>(A, ls, s_spa, s_spab, v2, v1, v0, cu, pp, fp, q, x0, A.amp2, bb);
+      store_row<VEC>(up2 + q * nx + x0, w);
+    }
+    v2 = v1;
+    v1 = v0;
+    pp = pc;
+    cu = cc;
+    cc = cn;
+    cn = nn;
+    mc = nm;
+    pc = np;
+    fp = fc;
+    fc = nf;
+    e1 = ne;
   }
 }
 
@@ -531,6 +761,14 @@ template <bool SPONGE, bool COV, int VEC, int RY, int PD, bool BATCH>
 __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<AdjArgs> L) {
   int lb = xcd_logical_block();
   const AdjArgs A = pick_item<BATCH>(L, lb);
+  __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
+  if (SPONGE) {
+    for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
+      s_spa[i] = A.sp_a[i];
+      s_spab[i] = A.sp_ab[i];
+    }
+    __syncthreads();
+  }
   const int nx = A.nx, ny = A.ny;
   const int tile = lb % A.tiles_per_shot, shot = lb / A.tiles_per_shot;
   const int x0 = (tile % A.tiles_x) * (kBlockX * VEC) + threadIdx.x * VEC;
@@ -548,6 +786,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<Ad
   float *__restrict__ cov = COV ? A.cov + gs * n2 : nullptr;
   const float *__restrict__ pw = A.pw;
   const int yend = min(y0 + RY, ny);
+  const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, SPONGE ? A.sponge_w : 0, s_spa, s_spab, xok);
 
   auto wok = [&](int y) { return xok && y >= y0 - 1 && y <= yend && y >= 0 && y < ny; };  // window rows
   auto sok = [&](int y) { return xok && y >= y0 && y < yend; };                            // strip rows
@@ -558,70 +797,82 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<Ad
   Row<VEC> zu = load_row<VEC>(z1 + at(y0 - 1, wok(y0 - 1)), wok(y0 - 1));
   Row<VEC> pu = load_row<VEC>(pw + at(y0 - 1, wok(y0 - 1)), wok(y0 - 1));
   Row<VEC> uu = load_row<VEC>(u + at(y0 - 1, wok(y0 - 1)), wok(y0 - 1));
-  Row<VEC> zc = load_row<VEC>(z1 + at(y0, wok(y0)), wok(y0));
-  Row<VEC> pc = load_row<VEC>(pw + at(y0, wok(y0)), wok(y0));
-  Row<VEC> uc = load_row<VEC>(u + at(y0, wok(y0)), wok(y0));
+  Row<VEC> zc = load_row_halo<VEC>(z1, y0, x0, nx, wok(y0));
+  Row<VEC> pc = load_row_halo<VEC>(pw, y0, x0, nx, wok(y0));
+  Row<VEC> uc = load_row_halo<VEC>(u, y0, x0, nx, wok(y0));
+  // receivers of a row (rare): the row's flag, then its cell map, read ahead
+  // with the row so that nothing is waited for when the row is consumed
+  unsigned rflags = 0;  // strip rows holding receivers, read once
+#pragma unroll
+  for (int i = 0; i < RY; ++i)
+    if (y0 + i < ny && A.row_has_recv[y0 + i]) rflags |= 1u << i;
+  auto recv_bits = [&](int y) {
+    int bits = 0;
+    if (sok(y) && ((rflags >> (y - y0)) & 1u)) {
+      int cm[VEC];
+      load_row_int<VEC>(A.cellmap + y * nx + x0, true, cm);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) bits |= (cm[e] >= 0) << e;
+    }
+    return bits;
+  };
   Row<VEC> qz[PD], qp[PD], qu[PD], q2[PD], qa[PD], qc[PD];
+  int qr[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) {
     const int yw = y0 + 1 + j, ys = y0 + j;
-    qz[j] = load_row<VEC>(z1 + at(yw, wok(yw)), wok(yw));
-    qp[j] = load_row<VEC>(pw + at(yw, wok(yw)), wok(yw));
-    qu[j] = load_row<VEC>(u + at(yw, wok(yw)), wok(yw));
+    qz[j] = load_row_halo<VEC>(z1, yw, x0, nx, wok(yw));
+    qp[j] = load_row_halo<VEC>(pw, yw, x0, nx, wok(yw));
+    qu[j] = load_row_halo<VEC>(u, yw, x0, nx, wok(yw));
     q2[j] = load_row<VEC>(z2 + at(ys, sok(ys)), sok(ys));
     qa[j] = load_row<VEC>(acc + at(ys, sok(ys)), sok(ys));
     if (COV) qc[j] = load_row<VEC>(cov + at(ys, sok(ys)), sok(ys));
+    qr[j] = recv_bits(ys);
   }
 
 #pragma unroll 1
   for (int base = y0; base < yend; base += PD) {
     Row<VEC> nz[PD], np[PD], nu[PD], n2r[PD], na[PD], nc[PD];
+    int nr[PD];
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int yw = base + PD + 1 + j, ys = base + PD + j;
-      nz[j] = load_row<VEC>(z1 + at(yw, wok(yw)), wok(yw));
-      np[j] = load_row<VEC>(pw + at(yw, wok(yw)), wok(yw));
-      nu[j] = load_row<VEC>(u + at(yw, wok(yw)), wok(yw));
+      nz[j] = load_row_halo<VEC>(z1, yw, x0, nx, wok(yw));
+      np[j] = load_row_halo<VEC>(pw, yw, x0, nx, wok(yw));
+      nu[j] = load_row_halo<VEC>(u, yw, x0, nx, wok(yw));
       n2r[j] = load_row<VEC>(z2 + at(ys, sok(ys)), sok(ys));
       na[j] = load_row<VEC>(acc + at(ys, sok(ys)), sok(ys));
       if (COV) nc[j] = load_row<VEC>(cov + at(ys, sok(ys)), sok(ys));
+      nr[j] = recv_bits(ys);
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int y = base + j;
       if (y < yend) {  // wavefront-uniform
         float zl, zr, pl, pr, ul, ur;
-        const int rb = xok ? y * nx : 0;
-        row_ends<VEC>(zc, z1 + rb, x0, nx, xok, zl, zr);
-        row_ends<VEC>(pc, pw + rb, x0, nx, xok, pl, pr);
-        row_ends<VEC>(uc, u + rb, x0, nx, xok, ul, ur);
+        halo_ends<VEC>(zc, zl, zr);
+        halo_ends<VEC>(pc, pl, pr);
+        halo_ends<VEC>(uc, ul, ur);
         if (xok && y > 0 && y < ny - 1) {
-          // receivers of this row (rare): one flag per row, then the row's cells
-          int skip = 0;
-          if (A.row_has_recv[y]) {
-            int cm[VEC];
-            load_row_int<VEC>(A.cellmap + y * nx + x0, true, cm);
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) skip |= (cm[e] >= 0) << e;
-          }
+          const int skip = qr[j];
           Row<VEC> zo, ao, co;
 #pragma unroll
           for (int e = 0; e < VEC; ++e) {
             const int x = x0 + e;
-            float a, ab;
-            sponge_factors<SPONGE>(A, x, y, a, ab);
+            float a = 1.f, ab = 1.f;
+            if (SPONGE) sponge_at<VEC>(ls, e, y, ny, A.sponge_w, s_spa, s_spab, a, ab);
             const float zle = e == 0 ? zl : zc.e[e - 1];
             const float zre = e == VEC - 1 ? zr : zc.e[e + 1];
             const float ple = e == 0 ? pl : pc.e[e - 1];
             const float pre = e == VEC - 1 ? pr : pc.e[e + 1];
             const float ule = e == 0 ? ul : uc.e[e - 1];
             const float ure = e == VEC - 1 ? ur : uc.e[e + 1];
-            const float zval = (((pu.e[e] * zu.e[e] + qp[j].e[e] * qz[j].e[e]) + ple * zle) + pre * zre) +
-                               (2.f * a - 4.f * pc.e[e]) * zc.e[e] - ab * q2[j].e[e];
-            const float lu = (((uu.e[e] + qu[j].e[e]) + ule) + ure - 4.f * uc.e[e]) * A.inv_h2;
+            const float zval = adjoint_rule(pu.e[e], zu.e[e], qp[j].e[e], qz[j].e[e], ple, zle, pre, zre, pc.e[e],
+                                            zc.e[e], a, ab, q2[j].e[e]);
+            const float lu = lap5(uu.e[e], qu[j].e[e], ule, ure, uc.e[e], A.inv_h2);
             zo.e[e] = zval;
-            ao.e[e] = qa[j].e[e] + zval * lu;
-            if (COV) co.e[e] = qc[j].e[e] + uc.e[e] * uc.e[e];
+            ao.e[e] = fused(zval, lu, qa[j].e[e]);
+            if (COV) co.e[e] = fused(uc.e[e], uc.e[e], qc[j].e[e]);
           }
           // side edges (backpropagation2): z = the inward cell's value; Lu with
           // clamped indices, the corner cells of rows 0 / ny-1 as the mean of
@@ -629,17 +880,17 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<Ad
           if (x0 == 0) {
             const float upv = y == 1 ? 0.5f * (uc.e[0] + uu.e[1]) : uu.e[0];
             const float dnv = y == ny - 2 ? 0.5f * (uc.e[0] + qu[j].e[1]) : qu[j].e[0];
-            const float lu = (((upv + dnv) + uc.e[0]) + uc.e[1] - 4.f * uc.e[0]) * A.inv_h2;
+            const float lu = lap5(upv, dnv, uc.e[0], uc.e[1], uc.e[0], A.inv_h2);
             zo.e[0] = zo.e[1];
-            ao.e[0] = qa[j].e[0] + zo.e[0] * lu;
+            ao.e[0] = fused(zo.e[0], lu, qa[j].e[0]);
           }
           if (x0 + VEC == nx) {
             constexpr int E = VEC - 1;
             const float upv = y == 1 ? 0.5f * (uc.e[E] + uu.e[E - 1]) : uu.e[E];
             const float dnv = y == ny - 2 ? 0.5f * (uc.e[E] + qu[j].e[E - 1]) : qu[j].e[E];
-            const float lu = (((upv + dnv) + uc.e[E - 1]) + uc.e[E] - 4.f * uc.e[E]) * A.inv_h2;
+            const float lu = lap5(upv, dnv, uc.e[E - 1], uc.e[E], uc.e[E], A.inv_h2);
             zo.e[E] = zo.e[E - 1];
-            ao.e[E] = qa[j].e[E] + zo.e[E] * lu;
+            ao.e[E] = fused(zo.e[E], lu, qa[j].e[E]);
           }
           const int id = y * nx + x0;
           if (!skip) {
@@ -669,6 +920,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<Ad
       q2[j] = n2r[j];
       qa[j] = na[j];
       if (COV) qc[j] = nc[j];
+      qr[j] = nr[j];
     }
   }
 }
@@ -697,8 +949,8 @@ __global__ __launch_bounds__(256) void adjoint_ring_kernel(Launch<AdjArgs> L) {
   float zval, lu, uval;
   adjoint_slow<SPONGE>(A, A.z1 + so, A.z2 + so, A.u + so, gs, x, y, c.y, zval, lu, uval);
   A.z0[so + c.x] = zval;
-  A.acc[gs * n2 + c.x] += zval * lu;
-  if (COV) A.cov[gs * n2 + c.x] += uval * uval;
+  A.acc[gs * n2 + c.x] = fused(zval, lu, A.acc[gs * n2 + c.x]);
+  if (COV) A.cov[gs * n2 + c.x] = fused(uval, uval, A.cov[gs * n2 + c.x]);
 }
 
 // ------------------------------------------------------------- elementwise
@@ -1243,6 +1495,42 @@ hipError_t launch_weights_only(const float *f, const float *G, int M, int nx, in
   const int chunk = (int)((n2 + part_blocks - 1) / part_blocks);
   weighted_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, n2, chunk, part);
   weighted_weights_kernel<<<1, 256, 0, st>>>(part, M, part_blocks, w);
+  return hipGetLastError();
+}
+
+bool forward2_supported(int nx, int sponge_w) {
+  return sponge_w > 0 && (nx == 128 || nx == 256 || nx == 512);
+}
+
+hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st) {
+  const int vec = a.nx / kBlockX;
+  const int ry = (rows == 8 || rows == 32) ? rows : 16;
+  const int strips = (a.ny + ry - 1) / ry;
+  a.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
+  const dim3 grid(a.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
+#define TOMO_FWD2(V)                                                                  \
+  switch (ry) {                                                                       \
+    case 8: forward2_kernel<V, 8><<<grid, block, 0, st>>>(a); break;                  \
+    case 32: forward2_kernel<V, 32><<<grid, block, 0, st>>>(a); break;                \
+    default: forward2_kernel<V, 16><<<grid, block, 0, st>>>(a); break;                \
+  }
+  if (vec == 8) {
+    TOMO_FWD2(8)
+  } else if (vec == 4) {
+    TOMO_FWD2(4)
+  } else if (vec == 2) {
+    TOMO_FWD2(2)
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef TOMO_FWD2
+  return hipGetLastError();
+}
+
+hipError_t launch_forward_ring(const FwdArgs<float> &a, int nshots, hipStream_t st) {
+  if (a.nring <= 0) return hipSuccess;
+  const Launch<FwdArgs<float>> L{a, nullptr, nullptr, 0};
+  forward_ring_kernel<true, false><<<dim3((a.nring + 255) / 256, nshots, 1), 256, 0, st>>>(L);
   return hipGetLastError();
 }
 

@@ -1,0 +1,146 @@
+"""Convergence studies of the benchmark configurations (run through
+``bench.py --study``).
+
+- ``c2``: non-uniform weighted SIRT with per-pixel coverage weights against
+  sequential ART on the 256^2 problem (64 transducers, 2000 steps, 3-6
+  inclusions, 160^2 imaging region), with additive Gaussian noise of 0 and
+  10 % of each trace's RMS; 30 iterations each at fixed relaxations (SIRT:
+  1/4 of the misfit-optimal first step; ART: the same per-shot step
+  divided by the shot count so one ART sweep moves the field as far as one
+  SIRT step).
+- ``ttt``: time to target image error, learned vs plain SIRT, on the 512^2,
+  128-source, 4000-step problem: plain batched SIRT runs 50 iterations and
+  sets the target (its final relative error); the learned run takes 3
+  learned iterations (the network of ``weights/preconditioner.npy``, with
+  its misfit-checked plain fallback) and continues with plain iterations
+  until it reaches the target.
+- ``train``: the training run that produced ``weights/preconditioner.npy``:
+  16 seeded 256^2 phantoms (seeds 7000-7015; 64 transducers, 2048 steps,
+  checkpoint 200), 40 plain-SIRT iterations each at relaxation 0.1 x the
+  misfit-optimal first step, every iterate a sample whose target is the
+  trajectory's best iterate (lowest error), 1500 Adam steps of batch 8 at
+  learning rate 1e-3 (seed 0).
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import numpy as np
+
+from . import UPDATE_MEAN, Preconditioner, add_noise
+from . import learned as L
+from . import workloads as W
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(PKG_DIR, "weights", "preconditioner.npy")
+
+
+def c2_study(iters: int = 30, seed: int = 0, log=None) -> dict:
+    c, ii, jj, fo = W.problem("c2", seed)
+    rows = []
+    for noise in (0.0, 0.10):
+        e = W.engine_for(c, ii, jj, fo, coverage=True)
+        if noise > 0:
+            e.set_field(fo)
+            meas = add_noise(e.forward(fp64=True), noise, 77 + seed)
+            e.set_field(np.zeros(fo.size, np.float32))
+            e.set_measured(meas)
+        # relaxation: misfit-optimal first step along the plain direction
+        m0 = e.gradient()
+        g = np.stack([e.get_gradient(s) for s in range(e.num_shots)]).astype(np.float64).sum(0) / e.num_shots
+        base = 0.01 / max(float(np.abs(g).max()), 1e-30)
+        best = (m0, base)
+        for k in range(12):
+            e.set_field((base * 2 ** k * g).astype(np.float32) * _mask(c))
+            m = e.misfit()
+            if m < best[0]:
+                best = (m, base * 2 ** k)
+            else:
+                break
+        scale = 0.25 * best[1]
+        for method in ("weighted+coverage", "art"):
+            e.set_field(np.zeros(fo.size, np.float32))
+            e.set_pixel_weights(1 if method != "art" else 0)
+            t0 = time.perf_counter()
+            eps = []
+            for _ in range(iters):
+                if method == "art":
+                    eps.append(e.art_iteration(scale / e.num_shots)[1])
+                else:
+                    eps.append(e.sirt_iteration(scale, weighted=True, mode=UPDATE_MEAN)[1])
+            secs = time.perf_counter() - t0
+            rows.append({"noise": noise, "method": method, "eps": [round(v, 3) for v in eps],
+                         "final_eps": round(eps[-1], 3), "best_eps": round(min(eps), 3), "seconds": round(secs, 2)})
+            if log:
+                log(f"c2 noise {noise:.0%} {method}: eps {eps[0]:.2f} -> {eps[-1]:.2f} (best {min(eps):.2f}) "
+                    f"in {secs:.1f} s")
+        e.close()
+    return {"study": "c2", "iterations": iters, "rows": rows}
+
+
+def _mask(c):
+    m = np.zeros((c.ny, c.nx), np.float32)
+    m[c.mask_y0:c.mask_y1, c.mask_x0:c.mask_x1] = 1.0
+    return m.ravel()
+
+
+def train_study(out: str, log=None) -> dict:
+    """Reproduce the shipped network (see the module docstring); writes the
+    parameters (float32, numpy .npy) to `out`."""
+    net = Preconditioner(device=0, seed=1)
+    t = time.perf_counter()
+    rep = L.train(net, 256, range(7000, 7016), 40, frac=0.1, rounds=0, train_steps=1500, batch=8, lr=1e-3,
+                  target="best", log=log)
+    np.save(out, net.get_params(), allow_pickle=False)
+    net.close()
+    return {"study": "train", "out": out, "phantoms": rep.phantoms, "samples": rep.samples,
+            "loss_first": rep.losses[0], "loss_last50": float(np.mean(rep.losses[-50:])),
+            "seconds": {k: round(v, 2) for k, v in rep.seconds.items()}, "total_seconds": round(time.perf_counter() - t, 2)}
+
+
+def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = 0.1, seed: int = 9100,
+              weights: str = WEIGHTS, log=None) -> dict:
+    """Plain batched SIRT for `plain_iters` iterations sets the target; the
+    learned run takes `learned_iters` learned iterations and continues with
+    plain iterations until it reaches the target (or twice the plain
+    iteration count)."""
+    n = 512
+    net = Preconditioner(device=0)
+    net.set_params(np.load(weights, allow_pickle=False))
+    p = L.Problem(n, L.phantom(n, seed), steps=4000, shots=128, checkpoint=200)
+    t = time.perf_counter()
+    scale = L.calibrate_scale(p, frac)
+    t_cal = time.perf_counter() - t
+    p.reset()
+    tp = L.run_plain(p, scale, plain_iters)
+    target = tp.eps[-1]
+    if log:
+        log(f"plain: {plain_iters} iterations in {tp.seconds[-1]:.1f} s, eps {tp.eps[0]:.2f} -> {target:.3f}")
+    p.reset()
+    t0 = time.perf_counter()
+    tl = L.run_learned(net, p, scale, learned_iters, target=target)
+    eps, secs, acc = list(tl.eps), list(tl.seconds), list(tl.accepted)
+    if eps[-1] > target:
+        base = time.perf_counter() - t0
+        tq = L.run_plain(p, scale, 2 * plain_iters, target=target)
+        eps += tq.eps
+        secs += [base + s for s in tq.seconds]
+        acc += [True] * len(tq.eps)
+    reached = next((i for i, e in enumerate(eps) if e <= target), None)
+    t_learned = secs[reached] if reached is not None else None
+    if log:
+        log(f"learned: {len(eps)} iterations, eps {eps[0]:.2f} -> {eps[-1]:.3f}, target reached at "
+            f"{t_learned} s (iteration {reached})")
+    p.close()
+    net.close()
+    return {
+        "study": "ttt", "grid": n, "shots": 128, "time_steps": 4000, "phantom_seed": seed,
+        "relaxation": scale, "calibration_seconds": round(t_cal, 2),
+        "target_eps": round(target, 4), "plain_seconds": round(tp.seconds[-1], 2),
+        "plain_eps": [round(v, 3) for v in tp.eps],
+        "learned_seconds": None if t_learned is None else round(t_learned, 2),
+        "learned_iterations": None if reached is None else reached + 1,
+        "learned_eps": [round(v, 3) for v in eps], "learned_steps_accepted": acc[:learned_iters],
+        "speedup": None if t_learned is None else round(tp.seconds[-1] / t_learned, 2),
+    }
