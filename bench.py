@@ -50,10 +50,12 @@ ADJ_BYTES = 20.0
 # the coefficient image is shared by every shot of a launch (4/shots B,
 # L2-resident); the two-step forward kernel reads u(k), u(k-1) and writes
 # u(k+1), u(k+2) once per two steps (8 B), the one-step kernel 12 B; the
-# adjoint kernel re-reads and re-writes the per-shot gradient each step
+# two-step adjoint reads z(k+1), z(k+2), u(k), u(k-1) and the per-shot
+# gradient and writes z(k), z(k-1) and the gradient once per two steps
+# (16 B), the one-step adjoint re-reads and re-writes the gradient each step
 # (24 B).
 FWD_LAYOUT_BYTES = {1: 12.0, 2: 8.0}
-ADJ_LAYOUT_BYTES = 24.0
+ADJ_LAYOUT_BYTES = {1: 24.0, 2: 16.0}
 # Measured HBM bytes per launch of the stencil kernels (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE of a --counter-pass run, scaled by the counters'
 # reading for a 1 GiB device copy).
@@ -132,7 +134,7 @@ def counter_pass(args):
     log("gradient")
     e.gradient()
     shape = list(e.get_shape())
-    tb = e.time_blocking()
+    tb, ab = e.time_blocking(), e.adjoint_blocking()
     e.close()
     log("copy 1 GiB")
     a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -140,7 +142,7 @@ def counter_pass(args):
     b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"counter_pass": "c3", "stencil_shape": shape, "forward_steps_per_launch": tb,
-                      "copy_bytes": 2 * a.numel() * 4}), flush=True)
+                      "adjoint_steps_per_launch": ab, "copy_bytes": 2 * a.numel() * 4}), flush=True)
 
 
 def main():
@@ -239,19 +241,20 @@ def main():
 
     # a forward launch advances every shot of the pass by `tb` levels; the
     # engine's spans count levels and cover each launch with its edge passes
-    tb = e.time_blocking()
+    tb, ab = e.time_blocking(), e.adjoint_blocking()
     shape = tuple(e.get_shape())
     fwd_cells_launch = shots * n * n * tb
-    adj_cells_launch = shots * n * n
+    adj_cells_launch = shots * n * n * ab
     fwd_avg_ms = tm["fwd_ms"] / max(tm["fwd_launches"], 1) * tb
-    adj_avg_ms = tm["adj_ms"] / max(tm["adj_launches"], 1)
+    adj_avg_ms = tm["adj_ms"] / max(tm["adj_launches"], 1) * ab
     fwd_gbs = FWD_BYTES * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
     adj_gbs = ADJ_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
     fwd_layout_gbs = FWD_LAYOUT_BYTES[tb] * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
-    adj_layout_gbs = ADJ_LAYOUT_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
+    adj_layout_gbs = ADJ_LAYOUT_BYTES[ab] * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
     fwd_kernel = "forward2_kernel" if tb == 2 else "forward_vec_kernel"
+    adj_kernel = "adjoint2_kernel" if ab == 2 else "adjoint_vec_kernel"
     fwd_traffic = load_traffic(fwd_kernel, shape, tb) if args.workload == "c3" else None
-    adj_traffic = load_traffic("adjoint_vec_kernel", shape, tb) if args.workload == "c3" else None
+    adj_traffic = load_traffic(adj_kernel, shape, ab) if args.workload == "c3" else None
 
     out = {
         "metric": "Gcell-updates/s of FDTD step",
@@ -271,29 +274,33 @@ def main():
                         f"checkpoint every {ckpt or 'none'}",
             "grid": n, "shots": shots, "time_steps": steps, "checkpoint_every": ckpt,
             "cell_updates_per_step": cells / args.steps, "stencil_shape": list(shape),
-            "forward_steps_per_launch": tb,
+            "forward_steps_per_launch": tb, "adjoint_steps_per_launch": ab,
             "eps_after_steps": eps[-1] if eps else None,
         },
-        "roofline": {
-            "bound": "hbm", "kernel": f"{fwd_kernel} (with its edge/receiver passes)",
-            "achieved": round(fwd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": fwd_traffic,
+        "roofline": None,
+        "forward": {
+            "kernel": f"{fwd_kernel} (with its edge/receiver passes)",
+            "achieved": round(fwd_gbs, 1), "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": fwd_traffic,
             "bytes_per_cell_update": FWD_BYTES, "cells_per_launch": fwd_cells_launch,
-            "avg_launch_ms": round(fwd_avg_ms, 5),
+            "avg_launch_ms": round(fwd_avg_ms, 5), "total_ms": round(tm["fwd_ms"], 2),
             "layout_bytes_per_cell_update": FWD_LAYOUT_BYTES[tb], "layout_achieved": round(fwd_layout_gbs, 1),
             "layout_frac": round(fwd_layout_gbs / HBM_PEAK_GBS, 4),
         },
         "adjoint": {
-            "kernel": "adjoint_vec_kernel (with its edge/receiver pass)", "achieved": round(adj_gbs, 1),
-            "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "traffic": adj_traffic,
+            "kernel": f"{adj_kernel} (with its edge/receiver pass)",
+            "achieved": round(adj_gbs, 1), "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "traffic": adj_traffic,
             "bytes_per_cell_update": ADJ_BYTES, "cells_per_launch": adj_cells_launch,
-            "avg_launch_ms": round(adj_avg_ms, 5),
-            "layout_bytes_per_cell_update": ADJ_LAYOUT_BYTES, "layout_achieved": round(adj_layout_gbs, 1),
+            "avg_launch_ms": round(adj_avg_ms, 5), "total_ms": round(tm["adj_ms"], 2),
+            "layout_bytes_per_cell_update": ADJ_LAYOUT_BYTES[ab], "layout_achieved": round(adj_layout_gbs, 1),
             "layout_frac": round(adj_layout_gbs / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": None,
     }
 
+    # the roofline line is the stencil kernel with the larger share of the
+    # timed region
+    dom = "adjoint" if tm["adj_ms"] > tm["fwd_ms"] else "forward"
+    out["roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", **out[dom]}
     if not args.no_cpu_baseline:
         import oracle as orc
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)

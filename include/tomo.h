@@ -326,14 +326,20 @@ int tomo_timing_reset(tomo_ctx *ctx);
  * sirt-mimo.cu:165-299). */
 int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth);
 
-/* Forward steps per launch in the gradient passes: 2 (default) runs two
+/* Stencil steps per launch in the gradient passes: 2 (default) runs two
  * leapfrog steps in one pass over each strip (the intermediate level kept in
- * registers: 8 instead of 12 bytes of traffic per cell-update) where it
- * applies -- the sponge preset, vectorised kernels, nx of 128, 256 or 512,
- * checkpoint interval >= 4 -- and 1 otherwise; results are the same.
- * tomo_get_time_blocking returns the steps per launch in effect. */
+ * registers: 8 instead of 12 bytes of traffic per forward cell-update) where
+ * it applies -- the sponge preset, vectorised kernels, nx of 128, 256 or
+ * 512, checkpoint interval >= 4 -- and 1 otherwise.  Where the forward pass
+ * runs two steps, the adjoint sweep does too (16 instead of 24 bytes per
+ * cell-update) when the receivers are points with additive injection, no
+ * 64th of a row holds two of them, and coverage is off.  Results are the
+ * same bit for bit.  tomo_get_time_blocking returns the forward steps per
+ * launch in effect, tomo_get_adjoint_blocking the adjoint's.  No reference
+ * counterpart (one launch per step, sirt-mimo.cu:291-299). */
 int tomo_set_time_blocking(tomo_ctx *ctx, int steps);
 int tomo_get_time_blocking(const tomo_ctx *ctx);
+int tomo_get_adjoint_blocking(const tomo_ctx *ctx);
 
 /* Autotuner: times the vectorised forward and adjoint kernels of this
  * context (its grid, shots per pass, sponge and coverage) over the shapes

@@ -1,9 +1,13 @@
-"""Two forward steps per launch (the intermediate level held in registers)
-against one step per launch: traces, misfits and gradients must be bitwise
-equal -- the per-cell arithmetic is the same -- for each full-row lane width
-(nx = 128, 256, 512 -> 2, 4, 8 cells per lane), with an odd number of steps
-(a single-step tail), checkpoint segments of odd and even length, and the
-edge/receiver passes around each pair."""
+"""Two stencil steps per launch (the intermediate level held in registers),
+forward and adjoint, against one step per launch: traces, misfits and
+gradients must be bitwise equal -- the per-cell arithmetic is the same -- for
+each full-row lane width (nx = 128, 256, 512 -> 2, 4, 8 cells per lane), with
+an odd number of steps (a single-step tail), checkpoint segments of odd and
+even length (adjoint pairs that stop short of a segment's first level), the
+edge/receiver passes around each pair, and receivers on strip and halo
+rows.  Where the two-step adjoint does not apply (coverage on, two receivers
+in one lane segment) the engine falls back to one-step adjoint launches with
+the same results."""
 import numpy as np
 import pytest
 
@@ -13,38 +17,85 @@ from _pkg import tomo
 pytestmark = pytest.mark.gpu
 
 
-def engine(n, nt, ckpt, steps, shots=6):
+def engine(n, nt, ckpt, steps, shots=6, coverage=False, receivers=None):
     c = unit_config(n, nt)
     c.checkpoint_every = ckpt
     ii, jj = tomo.transducers_ring(n, n, shots, 0.39 * n)
     e = tomo.Engine(c, device=0)
+    if coverage:
+        e.enable_coverage(True)
     e.set_transducers(ii, jj, one_based=False)
-    e.set_receivers(ii, jj, one_based=False)
+    ri, rj = (ii, jj) if receivers is None else receivers
+    e.set_receivers(ri, rj, one_based=False)
     e.set_groups(1)
     e.set_time_blocking(steps)
     return e, c
 
 
-@pytest.mark.parametrize("n,nt,ckpt", [(128, 301, 0), (256, 301, 49), (512, 203, 50)])
+def compare(e1, e2, c, shots, coverage=False):
+    fo = unit_phantom(c, seed=7)
+    e1.set_field(fo)
+    meas = e1.forward(fp64=True)
+    f = (0.5 * fo).astype(np.float32)
+    for e in (e1, e2):
+        e.set_measured(meas)
+        e.set_field(f)
+    m1, m2 = e1.gradient(), e2.gradient()
+    print(f"  misfit {m1!r} {m2!r}", flush=True)
+    assert m1 == m2
+    for s in range(shots):
+        np.testing.assert_array_equal(e2.get_gradient(s), e1.get_gradient(s))
+    if coverage:
+        np.testing.assert_array_equal(e2.get_coverage(), e1.get_coverage())
+
+
+@pytest.mark.parametrize("n,nt,ckpt", [(128, 301, 0), (256, 301, 49), (512, 203, 50), (512, 160, 7)])
 def test_two_step_launches_match_single_steps(n, nt, ckpt):
     print(f"[case] time blocking n={n} nt={nt} ckpt={ckpt}", flush=True)
     e1, c = engine(n, nt, ckpt, 1)
     e2, _ = engine(n, nt, ckpt, 2)
     try:
         assert e1.time_blocking() == 1 and e2.time_blocking() == 2
-        fo = unit_phantom(c, seed=7)
-        e1.set_field(fo)
-        meas = e1.forward(fp64=True)
-        f = (0.5 * fo).astype(np.float32)
-        for e in (e1, e2):
-            e.set_measured(meas)
-            e.set_field(f)
-        m1, m2 = e1.gradient(), e2.gradient()
-        print(f"  misfit {m1!r} {m2!r}", flush=True)
-        assert m1 == m2
-        for s in range(6):
-            g1, g2 = e1.get_gradient(s), e2.get_gradient(s)
-            np.testing.assert_array_equal(g2, g1)
+        assert e1.adjoint_blocking() == 1 and e2.adjoint_blocking() == 2
+        compare(e1, e2, c, 6)
+    finally:
+        e1.close()
+        e2.close()
+
+
+def test_many_receivers_on_strip_and_halo_rows():
+    """128 sources and receivers on the ring at 256^2 (receivers on most
+    rows, so on strip boundaries and halo rows of both levels)."""
+    print("[case] time blocking 256 with 128 receivers", flush=True)
+    e1, c = engine(256, 241, 60, 1, shots=128)
+    e2, _ = engine(256, 241, 60, 2, shots=128)
+    try:
+        assert e2.adjoint_blocking() == 2
+        compare(e1, e2, c, 128)
+    finally:
+        e1.close()
+        e2.close()
+
+
+def test_fallback_to_one_step_adjoint():
+    print("[case] adjoint fallback: coverage, crowded receivers", flush=True)
+    # coverage accumulation: forward pairs, one-step adjoint
+    e1, c = engine(256, 201, 40, 1, coverage=True)
+    e2, _ = engine(256, 201, 40, 2, coverage=True)
+    try:
+        assert e2.time_blocking() == 2 and e2.adjoint_blocking() == 1
+        compare(e1, e2, c, 6, coverage=True)
+    finally:
+        e1.close()
+        e2.close()
+    # two receivers in one 4-cell lane segment of a row
+    ri = np.array([100, 101, 150], np.int32)
+    rj = np.array([120, 120, 60], np.int32)
+    e1, c = engine(256, 201, 40, 1, receivers=(ri, rj))
+    e2, _ = engine(256, 201, 40, 2, receivers=(ri, rj))
+    try:
+        assert e2.time_blocking() == 2 and e2.adjoint_blocking() == 1
+        compare(e1, e2, c, 6)
     finally:
         e1.close()
         e2.close()

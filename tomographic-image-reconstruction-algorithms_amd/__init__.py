@@ -127,6 +127,7 @@ SIGNATURES = {
     "tomo_set_graphs": (_int, [_vp, _int]),
     "tomo_set_time_blocking": (_int, [_vp, _int]),
     "tomo_get_time_blocking": (_int, [_vp]),
+    "tomo_get_adjoint_blocking": (_int, [_vp]),
     "tomo_autotune": (_int, [_vp, _int, _i32p, _f64p]),
     "tomo_timing_graphs": (_int, [_vp, _f64p]),
     # batch of problems
@@ -453,6 +454,10 @@ class Engine:
 
     def time_blocking(self) -> int:
         return self._c(lib().tomo_get_time_blocking(self._ctx))
+
+    def adjoint_blocking(self) -> int:
+        """Adjoint steps per launch in effect (2 where the two-step adjoint applies)."""
+        return self._c(lib().tomo_get_adjoint_blocking(self._ctx))
 
     def autotune(self, force=False):
         """Time the stencil shapes on this context and keep the fastest; returns (shape, (fwd_ms, adj_ms))."""

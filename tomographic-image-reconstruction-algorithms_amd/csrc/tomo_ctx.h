@@ -47,6 +47,11 @@ struct tomo_ctx {
   int2 *d_ring = nullptr;                 // edge cells + receiver cells for the vectorised kernels
   int nring = 0;
   unsigned char *d_rowflag = nullptr;     // per row: holds a receiver
+  // receiver of each 64th of a row (the lane segments of the two-step
+  // adjoint, vec = nx / 64): receiver * vec + cell, or -1; recvseg_ok is
+  // false when a segment holds two receivers (the two-step adjoint is then off)
+  int *d_recvseg = nullptr;
+  bool recvseg_ok = false;
   int4 *d_bbox = nullptr;
   float *d_spa = nullptr, *d_spab = nullptr;
   double *d_spa64 = nullptr, *d_spab64 = nullptr;

@@ -64,6 +64,14 @@ StencilShape stencil_shape(const tomo_ctx *c) {
   return s;
 }
 
+// strip height of the two-step adjoint: the forward's rule (its halo is two
+// rows of z(k+1) per strip, its register window does not grow with it);
+// measured at 512^2 x 128 shots: 32 rows 5 % faster than 16
+int adjoint2_rows(const tomo_ctx *c) {
+  if (c->shape_req.adj_rows) return c->shape_req.adj_rows;
+  return c->ny >= 512 ? 32 : c->ny >= 256 ? 16 : 8;
+}
+
 bool depth_valid(int d) { return d == 0 || d == 1 || d == 2 || d == 4; }
 
 bool shape_valid(int vec, int fr, int ar) {
@@ -209,6 +217,21 @@ int upload_receivers(tomo_ctx *c) {
   HIPCHK(c, hipMemcpyAsync(c->d_ring, ring.data(), sizeof(int2) * ring.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_rowflag, rowflag.data(), c->ny, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->recvseg_ok = false;
+  if (adjoint2_supported(c->nx, c->cfg.sponge_width)) {
+    const int vec = c->nx / 64;
+    std::vector<int> seg((size_t)c->ny * 64, -1);
+    bool ok = true;
+    for (size_t r = 0; r < c->rcell.size(); ++r) {
+      const int x = c->rcell[r] % c->nx, y = c->rcell[r] / c->nx;
+      int &d = seg[(size_t)y * 64 + x / vec];
+      if (d >= 0) ok = false;
+      d = (int)r * vec + x % vec;
+    }
+    if ((rc = dalloc(c, &c->d_recvseg, seg.size()))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_recvseg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice, c->stream));
+    c->recvseg_ok = ok;
+  }
   if ((rc = dalloc(c, &c->d_anchor, std::max<size_t>(c->ranchor.size(), 1)))) return rc;
   if (!c->ranchor.empty())
     HIPCHK(c, hipMemcpyAsync(c->d_anchor, c->ranchor.data(), sizeof(int) * c->ranchor.size(),
@@ -275,9 +298,11 @@ int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
     if ((rc = dalloc(c, &c->d_ckpt, ckpt_levels * lvl))) return rc;
     c->ckpt_n = ckpt_levels * lvl;
   }
-  if (need_history && c->z_n < 3 * lvl) {
-    if ((rc = dalloc(c, &c->d_z, 3 * lvl))) return rc;
-    c->z_n = 3 * lvl;
+  // adjoint levels: a ring of 4 (two-step launches write z(k-1) while other
+  // strips still read z(k+2) on their halo rows)
+  if (need_history && c->z_n < 4 * lvl) {
+    if ((rc = dalloc(c, &c->d_z, 4 * lvl))) return rc;
+    c->z_n = 4 * lvl;
   }
   return TOMO_OK;
 }
@@ -288,6 +313,14 @@ bool two_step(const tomo_ctx *c) {
   if (c->time_block < 2 || !forward2_supported(c->nx, c->cfg.sponge_width)) return false;
   if (stencil_shape(c).vec < 2) return false;
   return full_history(c) || c->cfg.checkpoint_every >= 4;
+}
+
+// two adjoint steps per launch: with two forward steps per launch, point
+// receivers with additive injection, at most one receiver per lane segment,
+// and no coverage accumulation
+bool two_step_adjoint(const tomo_ctx *c) {
+  return two_step(c) && adjoint2_supported(c->nx, c->cfg.sponge_width) && c->recvseg_ok &&
+         c->cfg.recv_mode == TOMO_RECV_POINTS && !c->coverage;
 }
 
 // location of forward level k during the first (recording) pass (the
@@ -476,6 +509,27 @@ AdjArgs adj_args(tomo_ctx *c, int shot0) {
   return a;
 }
 
+Adj2Args adj2_args(tomo_ctx *c, int shot0) {
+  Adj2Args a{};
+  a.sstride = c->n2;
+  a.pw = c->d_pw;
+  a.recv_seg = c->d_recvseg;
+  a.rr = c->d_rr;
+  a.acc = c->d_grad;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nt = c->nt;
+  a.nr = c->nr;
+  a.shot0 = shot0;
+  a.gain = (float)c->cfg.recv_gain;
+  a.inv_h2 = (float)(1.0 / (c->cfg.h * c->cfg.h));
+  a.row_has_recv = c->d_rowflag;
+  a.sponge_w = c->cfg.sponge_width;
+  a.sp_a = c->d_spa;
+  a.sp_ab = c->d_spab;
+  return a;
+}
+
 // Gradient of shots [p0, p0+np) in one pass (np <= P).
 int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   const size_t tr = (size_t)c->nt * c->nr;
@@ -512,7 +566,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   // pass 2: adjoint sweep, newest segment first
   HIPCHK(c, launch_fill_f32(c->d_grad + (size_t)p0 * c->n2, (size_t)np * c->n2, 0.f, c->stream));
   if (c->coverage) HIPCHK(c, launch_fill_f32(c->d_cov + (size_t)p0 * c->n2, (size_t)np * c->n2, 0.f, c->stream));
-  auto Z = [&](int k) { return c->d_z + (size_t)(k % 3) * lvl; };
+  auto Z = [&](int k) { return c->d_z + (size_t)(k % 4) * lvl; };
   HIPCHK(c, launch_fill_f32(Z(nt - 1), (size_t)np * c->n2, 0.f, c->stream));
   HIPCHK(c, launch_fill_f32(Z(nt), (size_t)np * c->n2, 0.f, c->stream));
   const bool full = full_history(c);
@@ -522,6 +576,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   StencilShape ash = stencil_shape(c);
   ash.rows = ash.adj_rows;
   ash.depth = ash.adj_depth;
+  const bool two_adj = two_step_adjoint(c);
   for (int m = nseg - 1; m >= 0; --m) {
     const int k0 = m * C;
     const int cnt = std::min(C, nt - k0);
@@ -551,9 +606,27 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
     TimedSpan s;
     span_begin(c, s, 1);
     int launches = 0;
-    for (int q = cnt - 1; q >= 0; --q) {
+    for (int q = cnt - 1; q >= 0;) {
       const int k = k0 + q;
-      if (k < 1 || k > nt - 2) continue;
+      if (k < 1 || k > nt - 2) {
+        --q;
+        continue;
+      }
+      if (two_adj && q >= 1 && k - 1 >= 1) {  // levels k and k-1 of this segment
+        Adj2Args b = adj2_args(c, p0);
+        b.z1 = Z(k + 1);
+        b.z2 = Z(k + 2);
+        b.z0 = Z(k);
+        b.zm = Z(k - 1);
+        b.u0 = segment_level(c, P, m, k);
+        b.um = segment_level(c, P, m, k - 1);
+        b.level = k;
+        HIPCHK(c, launch_adjoint2(b, np, adjoint2_rows(c), c->stream));
+        HIPCHK(c, launch_adjoint2_edges(b, np, c->stream));
+        launches += 2;
+        q -= 2;
+        continue;
+      }
       aa.z1 = Z(k + 1);
       aa.z2 = Z(k + 2);
       aa.z0 = Z(k);
@@ -561,6 +634,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
       aa.level = k;
       HIPCHK(c, launch_adjoint_step(aa, np, ash, c->stream));
       ++launches;
+      --q;
     }
     s.launches = launches;
     s.cells = (double)launches * np * c->n2;
@@ -579,7 +653,8 @@ std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
                          (uintptr_t)c->d_rowflag, (uintptr_t)c->d_spa, (uintptr_t)c->d_spab,
                          (uintptr_t)c->coverage, (uintptr_t)c->nshots, (uintptr_t)c->nr, (uintptr_t)sh.vec,
                          (uintptr_t)sh.fwd_rows, (uintptr_t)sh.adj_rows, (uintptr_t)sh.fwd_depth,
-                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c)};
+                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c),
+                         (uintptr_t)two_step_adjoint(c), (uintptr_t)c->d_recvseg};
   return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
 }
 
@@ -809,6 +884,7 @@ void tomo_destroy(tomo_ctx *c) {
   graphs_clear(c);
   dfree(c->d_f); dfree(c->d_fo); dfree(c->d_pw); dfree(c->d_pw64);
   dfree(c->d_cellmap); dfree(c->d_anchor); dfree(c->d_bbox); dfree(c->d_ring); dfree(c->d_rowflag);
+  dfree(c->d_recvseg);
   dfree(c->d_spa); dfree(c->d_spab); dfree(c->d_spa64); dfree(c->d_spab64);
   dfree(c->d_meas); dfree(c->d_rr); dfree(c->d_grad); dfree(c->d_cov); dfree(c->d_misfit); dfree(c->d_pixw);
   dfree(c->d_scratch); dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_z); dfree(c->d_ring64);
@@ -1251,6 +1327,7 @@ int tomo_set_time_blocking(tomo_ctx *c, int steps) {
 }
 
 int tomo_get_time_blocking(const tomo_ctx *c) { return c ? (two_step(c) ? 2 : 1) : TOMO_ERR_ARG; }
+int tomo_get_adjoint_blocking(const tomo_ctx *c) { return c ? (two_step_adjoint(c) ? 2 : 1) : TOMO_ERR_ARG; }
 
 int tomo_set_pixel_weights(tomo_ctx *c, int mode, double eps) {
   int rc = check_ctx(c);

@@ -203,6 +203,17 @@ __device__ float lap_clamped(const float *__restrict__ u, int nx, int ny, int x,
               u_corner_mean(u, nx, ny, xb, y), u_corner_mean(u, nx, ny, x, y), inv_h2);
 }
 
+// Lu at any cell with clamped indices; a corner cell takes the mean of its
+// two edge neighbours' values (laplace_corners, sirt-mimo.cu:806-831)
+__device__ float edge_lu(const float *__restrict__ u, int nx, int ny, int x, int y, float inv_h2) {
+  const bool ex = (x == 0 || x == nx - 1), ey = (y == 0 || y == ny - 1);
+  if (ex && ey) {
+    const int xi = x == 0 ? 1 : nx - 2, yi = y == 0 ? 1 : ny - 2;
+    return (lap_clamped(u, nx, ny, xi, y, inv_h2) + lap_clamped(u, nx, ny, x, yi, inv_h2)) / 2.f;
+  }
+  return lap_clamped(u, nx, ny, x, y, inv_h2);
+}
+
 // Slow path: edge, corner and receiver cells.  Final z(k) follows
 // backpropagation2 (sirt-mimo.cu:712-774): receiver = inward anchor + gain *
 // residual, edge = copy of the inward cell, corner = mean of the two edge
@@ -223,11 +234,7 @@ __device__ void adjoint_slow(const AdjArgs &A, const float *__restrict__ z1, con
     zval = fused(A.rr[((long long)gs * A.nt + A.level) * A.nr + ri], A.gain,
                  adjoint_interior<SPONGE>(A, z1, z2, an % nx, an / nx));
   }
-  if (ex && ey) {
-    lu = (lap_clamped(u, nx, ny, xi, y, A.inv_h2) + lap_clamped(u, nx, ny, x, yi, A.inv_h2)) / 2.f;
-  } else {
-    lu = lap_clamped(u, nx, ny, x, y, A.inv_h2);
-  }
+  lu = edge_lu(u, nx, ny, x, y, A.inv_h2);
   uval = u_corner_mean(u, nx, ny, x, y);
 }
 
@@ -751,6 +758,31 @@ __global__ __launch_bounds__(256) void forward_ring_kernel(Launch<FwdArgs<float>
   }
 }
 
+// Side-edge cells of an adjoint row y in 1..ny-2 for the lanes holding them
+// (backpropagation2): z = the inward cell's value; Lu with clamped indices,
+// the corner cells of rows 0 / ny-1 read as the mean of their edge
+// neighbours (laplace, laplace_corners).  uu / uc / ud = u rows y-1, y, y+1.
+template <int VEC>
+__device__ __forceinline__ void adjoint_side_edges(const Row<VEC> &uu, const Row<VEC> &uc, const Row<VEC> &ud,
+                                                   const Row<VEC> &acc, Row<VEC> &zo, Row<VEC> &ao, int x0, int nx,
+                                                   int y, int ny, float inv_h2) {
+  if (x0 == 0) {
+    const float upv = y == 1 ? 0.5f * (uc.e[0] + uu.e[1]) : uu.e[0];
+    const float dnv = y == ny - 2 ? 0.5f * (uc.e[0] + ud.e[1]) : ud.e[0];
+    const float lu = lap5(upv, dnv, uc.e[0], uc.e[1], uc.e[0], inv_h2);
+    zo.e[0] = zo.e[1];
+    ao.e[0] = fused(zo.e[0], lu, acc.e[0]);
+  }
+  if (x0 + VEC == nx) {
+    constexpr int E = VEC - 1;
+    const float upv = y == 1 ? 0.5f * (uc.e[E] + uu.e[E - 1]) : uu.e[E];
+    const float dnv = y == ny - 2 ? 0.5f * (uc.e[E] + ud.e[E - 1]) : ud.e[E];
+    const float lu = lap5(upv, dnv, uc.e[E - 1], uc.e[E], uc.e[E], inv_h2);
+    zo.e[E] = zo.e[E - 1];
+    ao.e[E] = fused(zo.e[E], lu, acc.e[E]);
+  }
+}
+
 // Adjoint step, vectorised, same sliding-window structure: z(k+1), the
 // coefficients and u(k) stream through a three-row window; z(k+2) and the
 // accumulators are read for the current row only.  Rows 1..ny-2 including
@@ -874,24 +906,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<Ad
             ao.e[e] = fused(zval, lu, qa[j].e[e]);
             if (COV) co.e[e] = fused(uc.e[e], uc.e[e], qc[j].e[e]);
           }
-          // side edges (backpropagation2): z = the inward cell's value; Lu with
-          // clamped indices, the corner cells of rows 0 / ny-1 as the mean of
-          // their edge neighbours (laplace, laplace_corners)
-          if (x0 == 0) {
-            const float upv = y == 1 ? 0.5f * (uc.e[0] + uu.e[1]) : uu.e[0];
-            const float dnv = y == ny - 2 ? 0.5f * (uc.e[0] + qu[j].e[1]) : qu[j].e[0];
-            const float lu = lap5(upv, dnv, uc.e[0], uc.e[1], uc.e[0], A.inv_h2);
-            zo.e[0] = zo.e[1];
-            ao.e[0] = fused(zo.e[0], lu, qa[j].e[0]);
-          }
-          if (x0 + VEC == nx) {
-            constexpr int E = VEC - 1;
-            const float upv = y == 1 ? 0.5f * (uc.e[E] + uu.e[E - 1]) : uu.e[E];
-            const float dnv = y == ny - 2 ? 0.5f * (uc.e[E] + qu[j].e[E - 1]) : qu[j].e[E];
-            const float lu = lap5(upv, dnv, uc.e[E - 1], uc.e[E], uc.e[E], A.inv_h2);
-            zo.e[E] = zo.e[E - 1];
-            ao.e[E] = fused(zo.e[E], lu, qa[j].e[E]);
-          }
+          adjoint_side_edges<VEC>(uu, uc, qu[j], qa[j], zo, ao, x0, nx, y, ny, A.inv_h2);
           const int id = y * nx + x0;
           if (!skip) {
             store_row<VEC>(z0 + id, zo);
@@ -951,6 +966,221 @@ __global__ __launch_bounds__(256) void adjoint_ring_kernel(Launch<AdjArgs> L) {
   A.z0[so + c.x] = zval;
   A.acc[gs * n2 + c.x] = fused(zval, lu, A.acc[gs * n2 + c.x]);
   if (COV) A.cov[gs * n2 + c.x] = fused(uval, uval, A.cov[gs * n2 + c.x]);
+}
+
+// One adjoint row by the interior rule, for full-row waves (the x-neighbours
+// of every lane come by DPP; the side-edge cells get the rule's value and are
+// overwritten by the caller).  pn / zn, ps / zs = coefficient and z(k+1)
+// rows above and below, pc / zc the row itself, z2 = z(k+2) at the row.
+template <int VEC>
+__device__ __forceinline__ Row<VEC> adjoint_row(const LaneSponge<VEC> &ls, const float *s_spa, const float *s_spab,
+                                                const Row<VEC> &pn, const Row<VEC> &zn, const Row<VEC> &ps,
+                                                const Row<VEC> &zs, const Row<VEC> &pc, const Row<VEC> &zc,
+                                                const Row<VEC> &z2, int y, int ny, int w) {
+  const float zl = from_prev_lane(zc.e[VEC - 1]), zr = from_next_lane(zc.e[0]);
+  const float pl = from_prev_lane(pc.e[VEC - 1]), pr = from_next_lane(pc.e[0]);
+  Row<VEC> out;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    float a, ab;
+    sponge_at<VEC>(ls, e, y, ny, w, s_spa, s_spab, a, ab);
+    const float zle = e == 0 ? zl : zc.e[e - 1];
+    const float zre = e == VEC - 1 ? zr : zc.e[e + 1];
+    const float ple = e == 0 ? pl : pc.e[e - 1];
+    const float pre = e == VEC - 1 ? pr : pc.e[e + 1];
+    out.e[e] = adjoint_rule(pn.e[e], zn.e[e], ps.e[e], zs.e[e], ple, zle, pre, zre, pc.e[e], zc.e[e], a, ab, z2.e[e]);
+  }
+  return out;
+}
+
+// Correlation of one row: acc + z * Lu for every cell (interior Laplacian of
+// the u rows above / at / below), then the side-edge cells' copies and
+// clamped Laplacians.
+template <int VEC>
+__device__ __forceinline__ Row<VEC> correlate_row(const Row<VEC> &uu, const Row<VEC> &uc, const Row<VEC> &ud,
+                                                  const Row<VEC> &acc, Row<VEC> &z, int x0, int nx, int y, int ny,
+                                                  float inv_h2) {
+  const float ul = from_prev_lane(uc.e[VEC - 1]), ur = from_next_lane(uc.e[0]);
+  Row<VEC> out;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    const float ule = e == 0 ? ul : uc.e[e - 1];
+    const float ure = e == VEC - 1 ? ur : uc.e[e + 1];
+    out.e[e] = fused(z.e[e], lap5(uu.e[e], ud.e[e], ule, ure, uc.e[e], inv_h2), acc.e[e]);
+  }
+  adjoint_side_edges<VEC>(uu, uc, ud, acc, z, out, x0, nx, y, ny, inv_h2);
+  return out;
+}
+
+// additive injection at the lane's receiver cell (descriptor d = receiver *
+// VEC + cell, or -1), as adjoint_slow does for a point receiver
+template <int VEC>
+__device__ __forceinline__ void inject(Row<VEC> &z, int d, float res, float gain) {
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) z.e[e] = (d >= 0 && d % VEC == e) ? fused(res, gain, z.e[e]) : z.e[e];
+}
+
+template <int VEC>
+__device__ __forceinline__ void side_copies(Row<VEC> &z, int x0, int nx) {
+  if (x0 == 0) z.e[0] = z.e[1];
+  if (x0 + VEC == nx) z.e[VEC - 1] = z.e[VEC - 2];
+}
+
+// Each lane walks rows r = rs .. qb+1 of its strip: it forms z(k) at row r
+// from the z(k+1) window (rows r-1, r, r+1) and z(k+2) row r, injects the
+// residual at receivers, stores it and correlates it with Lu(k) on the
+// strip's own rows; then it forms z(k-1) at row r-1 from the z(k) window held
+// in registers (rows r-2, r-1, r) and z(k+1) row r-1, and correlates it with
+// Lu(k-1) into the same gradient row, read once and written once.  z(k) is
+// formed on one halo row above and below the strip.  Row 0 of z(k) is the
+// copy of row 1 and row ny-1 the copy of row ny-2 (backpropagation2), so the
+// window needs nothing from outside the strip's rows and halo.  Per two
+// cell-updates: reads z(k+1), z(k+2), u(k), u(k-1), the gradient, writes z(k),
+// z(k-1), the gradient (16 B per cell-update instead of 24).  The arithmetic
+// of every cell is that of adjoint_vec_kernel and adjoint_ring_kernel, so the
+// fields and gradients agree with two one-step launches bit for bit.
+template <int VEC, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) {
+  const int nx = A.nx, ny = A.ny;
+  __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
+  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
+    s_spa[i] = A.sp_a[i];
+    s_spab[i] = A.sp_ab[i];
+  }
+  __syncthreads();
+  const int lb = xcd_logical_block();
+  const int shot = lb / A.blocks_per_shot;
+  const int y0 = ((lb % A.blocks_per_shot) * kBlockY + threadIdx.y) * RY;
+  const int yend = min(y0 + RY, ny);
+  const int qa = max(y0, 1), qb = min(yend, ny - 1) - 1;  // own rows of the interior rule
+  if (y0 >= ny || qa > qb) return;                          // whole wavefront
+  const int x0 = threadIdx.x * VEC;
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
+  const long long n2 = (long long)nx * ny;
+  const float *__restrict__ z1 = A.z1 + so;
+  const float *__restrict__ z2 = A.z2 + so;
+  const float *__restrict__ u0 = A.u0 + so;
+  const float *__restrict__ um = A.um + so;
+  float *__restrict__ z0 = A.z0 + so;
+  float *__restrict__ zm = A.zm + so;
+  float *__restrict__ acc = A.acc + gs * n2;
+  const float *__restrict__ pw = A.pw;
+  const float *__restrict__ rrk = A.rr + ((long long)gs * A.nt + A.level) * A.nr;  // residuals at level k
+  const float *__restrict__ rrm = rrk - A.nr;                                        // and at level k-1
+  const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
+  auto ld = [&](const float *__restrict__ base, int y, bool ok) {
+    ok = ok && y >= 0 && y < ny;
+    return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
+  };
+  // z(k) by the rule on rows [rs, re]
+  const int rs = max(qa - 1, 1), re = min(qb + 1, ny - 2);
+  auto own = [&](int y) { return y >= qa && y <= qb; };
+  auto lrow = [&](int y) { return y >= qa - 1 && y <= qb + 1; };  // u rows of the Laplacians
+  unsigned long long rflags = 0;                                     // rows rs .. re holding receivers
+#pragma unroll
+  for (int i = 0; i < RY + 2; ++i)
+    if (rs + i <= re && A.row_has_recv[rs + i]) rflags |= 1ull << i;
+  auto desc = [&](int y) {
+    const bool on = y >= rs && y <= re && ((rflags >> (y - rs)) & 1ull);
+    return on ? A.recv_seg[y * kBlockX + threadIdx.x] : -1;
+  };
+  auto resid = [&](int d, const float *__restrict__ r) { return d >= 0 ? r[d / VEC] : 0.f; };
+
+  Row<VEC> zu = ld(z1, rs - 1, true), zc = ld(z1, rs, true), zn = ld(z1, rs + 1, true);
+  Row<VEC> pp = ld(pw, rs - 2, false), pu = ld(pw, rs - 1, true), pc = ld(pw, rs, true), pn = ld(pw, rs + 1, true);
+  Row<VEC> uu = ld(u0, rs - 1, lrow(rs - 1)), uc = ld(u0, rs, lrow(rs)), un = ld(u0, rs + 1, lrow(rs + 1));
+  Row<VEC> m2 = ld(um, rs - 2, false), m1 = ld(um, rs - 1, lrow(rs - 1)), m0 = ld(um, rs, lrow(rs));
+  Row<VEC> zz = ld(z2, rs, true);
+  Row<VEC> ac = ld(acc, rs, own(rs)), aq = ld(acc, 0, false);
+  Row<VEC> v2 = aq, v1 = aq;  // z(k) rows r-2, r-1
+  int dr = desc(rs), dn = desc(rs + 1), dq = -1;
+  float rk = resid(dr, rrk), rkm = resid(dr, rrm), rq = 0.f;
+
+#pragma unroll 1
+  for (int r = rs; r <= qb + 1; ++r) {
+    // the next row's inputs are in flight while this row computes
+    const Row<VEC> zn2 = ld(z1, r + 2, r + 1 <= re);
+    const Row<VEC> pn2 = ld(pw, r + 2, r + 1 <= re);
+    const Row<VEC> un2 = ld(u0, r + 2, lrow(r + 2));
+    const Row<VEC> mn = ld(um, r + 1, lrow(r + 1));
+    const Row<VEC> zzn = ld(z2, r + 1, r + 1 <= re);
+    const Row<VEC> an = ld(acc, r + 1, own(r + 1));
+    const int d2 = desc(r + 2);
+    // z(k) at row r
+    Row<VEC> v0;
+    if (r <= re) {
+      v0 = adjoint_row<VEC>(ls, s_spa, s_spab, pu, zu, pn, zn, pc, zc, zz, r, ny, A.sponge_w);
+      inject<VEC>(v0, dr, rk, A.gain);
+      if (own(r)) {
+        ac = correlate_row<VEC>(uu, uc, un, ac, v0, x0, nx, r, ny, A.inv_h2);
+        store_row<VEC>(z0 + r * nx + x0, v0);
+      } else {
+        side_copies<VEC>(v0, x0, nx);
+      }
+    } else {
+      v0 = v1;  // row ny-1: the copy of row ny-2
+    }
+    if (r == 1) v1 = v0;  // row 0: the copy of row 1
+    // z(k-1) at row q = r-1
+    const int q = r - 1;
+    if (own(q)) {
+      Row<VEC> w = adjoint_row<VEC>(ls, s_spa, s_spab, pp, v2, pc, v0, pu, v1, zu, q, ny, A.sponge_w);
+      inject<VEC>(w, dq, rq, A.gain);
+      const Row<VEC> ao = correlate_row<VEC>(m2, m1, m0, aq, w, x0, nx, q, ny, A.inv_h2);
+      store_row<VEC>(zm + q * nx + x0, w);
+      store_row<VEC>(acc + q * nx + x0, ao);
+    }
+    // residuals of row r+1 (its descriptor was read a row ago)
+    const float rkn = resid(dn, rrk), rkmn = resid(dn, rrm);
+    aq = ac;
+    ac = an;
+    v2 = v1;
+    v1 = v0;
+    pp = pu;
+    pu = pc;
+    pc = pn;
+    pn = pn2;
+    zu = zc;
+    zc = zn;
+    zn = zn2;
+    uu = uc;
+    uc = un;
+    un = un2;
+    m2 = m1;
+    m1 = m0;
+    m0 = mn;
+    zz = zzn;
+    dq = dr;
+    dr = dn;
+    dn = d2;
+    rq = rkm;
+    rk = rkn;
+    rkm = rkmn;
+  }
+}
+
+// Rows 0 and ny-1 of z(k) and z(k-1) after adjoint2_kernel: copies of rows 1
+// and ny-2 (equal to adjoint_slow's inward-cell rule there, which has no
+// receivers), and their correlation with the clamped Laplacians, level k
+// first (the order of two one-step launches).
+__global__ __launch_bounds__(256) void adjoint2_edge_kernel(Adj2Args A) {
+  const int nx = A.nx, ny = A.ny;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 2 * nx) return;
+  const int shot = blockIdx.y, gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
+  const long long n2 = (long long)nx * ny;
+  const int x = i % nx, y = i < nx ? 0 : ny - 1, yi = y == 0 ? 1 : ny - 2;
+  const int id = y * nx + x, idi = yi * nx + x;
+  float a = A.acc[gs * n2 + id];
+  const float zk = A.z0[so + idi];
+  A.z0[so + id] = zk;
+  a = fused(zk, edge_lu(A.u0 + so, nx, ny, x, y, A.inv_h2), a);
+  const float zkm = A.zm[so + idi];
+  A.zm[so + id] = zkm;
+  a = fused(zkm, edge_lu(A.um + so, nx, ny, x, y, A.inv_h2), a);
+  A.acc[gs * n2 + id] = a;
 }
 
 // ------------------------------------------------------------- elementwise
@@ -1524,6 +1754,38 @@ hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st) {
     return hipErrorInvalidValue;
   }
 #undef TOMO_FWD2
+  return hipGetLastError();
+}
+
+bool adjoint2_supported(int nx, int sponge_w) { return forward2_supported(nx, sponge_w); }
+
+hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st) {
+  const int vec = a.nx / kBlockX;
+  const int ry = (rows == 8 || rows == 32) ? rows : 16;
+  const int strips = (a.ny + ry - 1) / ry;
+  a.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
+  const dim3 grid(a.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
+#define TOMO_ADJ2(V)                                                                  \
+  switch (ry) {                                                                       \
+    case 8: adjoint2_kernel<V, 8><<<grid, block, 0, st>>>(a); break;                  \
+    case 32: adjoint2_kernel<V, 32><<<grid, block, 0, st>>>(a); break;                \
+    default: adjoint2_kernel<V, 16><<<grid, block, 0, st>>>(a); break;                \
+  }
+  if (vec == 8) {
+    TOMO_ADJ2(8)
+  } else if (vec == 4) {
+    TOMO_ADJ2(4)
+  } else if (vec == 2) {
+    TOMO_ADJ2(2)
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef TOMO_ADJ2
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint2_edges(const Adj2Args &a, int nshots, hipStream_t st) {
+  adjoint2_edge_kernel<<<dim3((2 * a.nx + 255) / 256, nshots, 1), 256, 0, st>>>(a);
   return hipGetLastError();
 }
 
