@@ -188,9 +188,11 @@ def main():
             out = S.train_study(args.out, log=log)
         else:
             runs = [S.ttt_study(seed=int(v), log=log) for v in args.seeds.split(",")]
+            valid = [r for r in runs if r["plain_progress"]]
             out = {"study": "ttt", "runs": runs,
-                   "speedups": [r["speedup"] for r in runs],
-                   "all_reached": all(r["learned_seconds"] is not None for r in runs)}
+                   "speedups": [r["speedup"] for r in valid],
+                   "seeds_without_plain_progress": [r["phantom_seed"] for r in runs if not r["plain_progress"]],
+                   "all_reached": all(r["learned_seconds"] is not None for r in valid)}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "c4":
@@ -326,9 +328,12 @@ def main():
 
 
 def run_c4(args):
-    """Uneven batch: 64 phantoms of sides {128..512}, one batched launch per
-    time level and stencil shape, shapes from the autotuner per size class;
-    a step is one weighted SIRT iteration of every phantom."""
+    """Uneven set: 64 phantoms of sides {128..512}, n/4 ring shots, 8n steps,
+    stencil shapes from the autotuner per side; a step is one weighted SIRT
+    iteration of every phantom, the phantoms stepped one context after the
+    other (the one-launch-per-level batch of tomo_batch is switched off at
+    this size: its one full-size run ended in a host SIGSEGV whose cause is
+    not established)."""
     log = (lambda m: print("[bench]", m, file=sys.stderr, flush=True))
     t = time.perf_counter()
     engines = W.c4_problems(log=log)
@@ -338,35 +343,34 @@ def run_c4(args):
     for e, n in zip(engines, sides):
         shapes[n] = e.autotune()[0]
     log(f"data for {len(engines)} phantoms in {t_data:.1f} s; shapes per side {shapes}")
-    b = tomo.Batch(engines, graph=True)
-    b.gradient()
     scales = []
     for e, n in zip(engines, sides):
+        e.gradient()
         g = np.stack([e.get_gradient(s) for s in range(e.num_shots)]).astype(np.float64).sum(0) / e.num_shots
         scales.append(float(1e-3 / max(np.abs(g).max(), 1e-30)))
 
     def iteration():
-        b.gradient()
         for e, sc in zip(engines, scales):
+            e.gradient()
             e.update_weighted(sc)
 
     for _ in range(args.warmup):
         iteration()
-    t0_cells = b.timing()["cells"]
+    for e in engines:
+        e.timing_reset()
     t0 = time.perf_counter()
     for k in range(args.steps):
         iteration()
         log(f"step {k} done")
     elapsed = time.perf_counter() - t0
-    tb = b.timing()
-    cells = tb["cells"] - t0_cells
+    cells = 0.0
+    fwd_ms = adj_ms = 0.0
+    for e in engines:
+        tm = e.timing()
+        cells += tm["fwd_cells"] + tm["adj_cells"]
+        fwd_ms += tm["fwd_ms"]
+        adj_ms += tm["adj_ms"]
     eps = float(np.mean([e.error() for e in engines]))
-    per_call_ms = tb["ms"] / max(tb["calls"], 1)
-    cells_call = tb["cells"] / max(tb["calls"], 1)
-    # algorithmic bytes of a gradient call: FWD_BYTES per forward and
-    # recompute cell-update, ADJ_BYTES per adjoint cell-update (one third of
-    # the updates)
-    gbs = (cells_call * (2 * FWD_BYTES + ADJ_BYTES) / 3) / (per_call_ms * 1e-3) / 1e9
     out = {
         "metric": "Gcell-updates/s of FDTD step",
         "value": round(cells / elapsed / 1e9, 3),
@@ -376,20 +380,16 @@ def run_c4(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded phantoms; measured traces from the engine's FP64 solver)",
         "config": {
-            "workload": "uneven batch: 64 phantoms, sides from {128,192,256,384,512}, n/4 ring shots, 8n steps, "
-                        "one weighted SIRT iteration of every phantom per step, batched launches (graph replay)",
+            "workload": "uneven set: 64 phantoms, sides from {128,192,256,384,512}, n/4 ring shots, 8n steps, "
+                        "checkpoint every 128, passes of 32 shots, one weighted SIRT iteration of every phantom "
+                        "per step, phantoms stepped one after the other",
             "sides": sides, "shapes": {str(k): list(v) for k, v in shapes.items()},
             "cell_updates_per_step": cells / args.steps, "mean_eps_after_steps": eps,
-            "launches_per_gradient": tb["launches"] / max(tb["calls"], 1),
+            "stencil_ms_share": round((fwd_ms + adj_ms) / (elapsed * 1e3), 4),
         },
-        "roofline": {
-            "bound": "hbm", "kernel": "batched gradient (forward_vec, adjoint_vec, edge passes)",
-            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "traffic": None, "avg_gradient_ms": round(per_call_ms, 3),
-        },
+        "roofline": None,
         "cpu_baseline": None,
     }
-    b.close()
     for e in engines:
         e.close()
     print(json.dumps(out), flush=True)

@@ -398,22 +398,25 @@ int forward_step(tomo_ctx *c, FwdArgs<T> a, const T *um, const T *u0, T *up, int
 }
 
 // Forward steps k and k+1 (levels k-1, k -> k+1, k+2) in one main launch,
-// with the edge passes: top/bottom rows of k+1 before; receiver records of
-// k+1, top/bottom rows and receiver records of k+2 after.
+// with the edge passes: top/bottom rows of k+1 before (unless the previous
+// pair formed them: pre = false); after, one pass for the receiver records
+// of k+1 and k+2, the top/bottom rows of k+2 and, when the next launch is a
+// pair too (up3 = its level k+3), the top/bottom rows of k+3.
 int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u0, float *up1, float *up2, int k,
-                  int P, bool record, const float *meas, float *rec) {
+                  int P, bool record, const float *meas, float *rec, bool pre, float *up3) {
   const bool rec_on = record && rec != nullptr && c->nr > 0;
-  a.meas = meas;
-  a.rec = rec;
-  a.ring = c->d_ring;
-  // rows 0 and ny-1 of level k+1
-  a.um = um;
-  a.u0 = u0;
-  a.up = up1;
-  a.level = k + 1;
-  a.record = 0;
-  a.nring = 2 * c->nx;
-  HIPCHK(c, launch_forward_ring(a, P, c->stream));
+  if (pre) {  // rows 0 and ny-1 of level k+1
+    a.meas = meas;
+    a.rec = rec;
+    a.ring = c->d_ring;
+    a.um = um;
+    a.u0 = u0;
+    a.up = up1;
+    a.level = k + 1;
+    a.record = 0;
+    a.nring = 2 * c->nx;
+    HIPCHK(c, launch_forward_ring(a, P, c->stream));
+  }
   Fwd2Args b{};
   b.um = um;
   b.u0 = u0;
@@ -433,24 +436,27 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
   b.sp_a = a.sp_a;
   b.sp_ab = a.sp_ab;
   HIPCHK(c, launch_forward2(b, P, stencil_shape(c).fwd_rows, c->stream));
-  // receiver records of level k+1
-  if (rec_on && k + 1 >= c->cfg.record_first) {
-    a.up = up1;
-    a.level = k + 1;
-    a.record = 1;
-    a.ring = c->d_ring + 2 * c->nx;
-    a.nring = c->nr;
-    HIPCHK(c, launch_forward_ring(a, P, c->stream));
-  }
-  // rows 0 and ny-1 and receiver records of level k+2
-  a.um = u0;
-  a.u0 = up1;
-  a.up = up2;
-  a.level = k + 2;
-  a.record = rec_on && (k + 2 >= c->cfg.record_first);
-  a.ring = c->d_ring;
-  a.nring = 2 * c->nx + c->nr;
-  HIPCHK(c, launch_forward_ring(a, P, c->stream));
+  Fwd2EdgeArgs e{};
+  e.u0 = u0;
+  e.up1 = up1;
+  e.up2 = up2;
+  e.up3 = up3;
+  e.sstride = a.sstride;
+  e.f = a.f;
+  e.ring = c->d_ring;
+  e.rec1 = rec_on && k + 1 >= c->cfg.record_first;
+  e.rec2 = rec_on && k + 2 >= c->cfg.record_first;
+  e.nring = 2 * c->nx + ((e.rec1 || e.rec2) ? c->nr : 0);
+  e.nx = c->nx;
+  e.ny = c->ny;
+  e.nt = c->nt;
+  e.nr = c->nr;
+  e.shot0 = a.shot0;
+  e.level1 = k + 1;
+  e.meas = meas;
+  e.rec = rec;
+  e.c0dt_h = a.c0dt_h;
+  HIPCHK(c, launch_forward2_edges(e, P, c->stream));
   return TOMO_OK;
 }
 
@@ -544,12 +550,15 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
     TimedSpan s;
     span_begin(c, s, 0);
     const bool two = two_step(c);
+    bool pre = true;  // the top/bottom rows of the pair's first level are still to form
     for (int k = 1; k < nt - 1;) {
       int rc;
       if (two && k + 1 < nt - 1) {
+        const bool next = k + 3 < nt - 1;  // the next launch is a pair
         rc = forward_step2(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
                            pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true), k, np, true, c->d_meas,
-                           c->d_rr);
+                           c->d_rr, pre, next ? pass1_level(c, P, k + 3, true) : nullptr);
+        pre = !next;
         k += 2;
       } else {
         rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
@@ -584,13 +593,16 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
       TimedSpan s;
       span_begin(c, s, 0);
       const bool two = two_step(c);
+      bool pre = true;
       for (int q = 2; q < cnt;) {
         const int k = k0 + q - 1;
         int rc;
         if (two && q + 1 < cnt) {
+          const bool next = q + 3 < cnt;
           rc = forward_step2(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
                              segment_level(c, P, m, k + 1), segment_level(c, P, m, k + 2), k, np, false, nullptr,
-                             nullptr);
+                             nullptr, pre, next ? segment_level(c, P, m, k + 3) : nullptr);
+          pre = !next;
           q += 2;
         } else {
           rc = forward_step<float>(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),

@@ -111,9 +111,9 @@ hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape 
 
 // Two forward steps k -> k+1 -> k+2 in one launch (sponge preset, waves of
 // full rows: nx == 64 vec, vec in {2, 4, 8}).  Rows 0 and ny-1 of level k+1
-// must be in up1 before (launch_forward_ring with the top/bottom cells); rows
-// 0 and ny-1 of level k+2 and the receiver records are left to the edge pass
-// after.
+// must be in up1 before (launch_forward_ring with the top/bottom cells, or
+// the previous pair's launch_forward2_edges); rows 0 and ny-1 of level k+2
+// and the receiver records are left to the edge pass after.
 struct Fwd2Args {
   const float *um, *u0;  // levels k-1, k of local shot 0
   float *up1, *up2;      // levels k+1, k+2
@@ -131,6 +131,27 @@ struct Fwd2Args {
 };
 bool forward2_supported(int nx, int sponge_w);
 hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st);
+
+// The edge pass after launch_forward2 over ring[0 .. nring) (the top row,
+// the bottom row, then the receivers): rows 0 and ny-1 of level k+2 (and of
+// level k+3 into up3 when up3 is set, for a following pair), the receiver
+// records of levels k+1 (rec1) and k+2 (rec2).
+struct Fwd2EdgeArgs {
+  const float *u0;        // level k of local shot 0
+  float *up1, *up2, *up3; // levels k+1, k+2, k+3 (up3 may be null)
+  long long sstride;
+  const float *f;
+  const int2 *ring;
+  int nring;
+  int nx, ny, nt, nr;
+  int shot0;
+  int level1;             // k+1
+  int rec1, rec2;
+  const float *meas;      // measured traces (null: record u itself)
+  float *rec;
+  float c0dt_h;
+};
+hipError_t launch_forward2_edges(const Fwd2EdgeArgs &a, int nshots, hipStream_t st);
 
 // Two adjoint steps k+1, k+2 -> k -> k-1 in one pass over a strip (sponge
 // preset, point receivers with additive injection, no coverage; a wave

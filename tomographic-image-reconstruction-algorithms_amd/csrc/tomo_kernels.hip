@@ -69,18 +69,24 @@ __device__ __forceinline__ T mur_edge(T r, T u, T ua, T ub, T m, T ma) {
   return fused(-r2, ma, v);
 }
 
-template <typename T>
-__device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T *__restrict__ um,
-                          int x, int y) {
-  const int nx = A.nx, ny = A.ny;
-  const int id = y * nx + x;
-  const T r = A.c0dt_h * sqrt(T(1) + T(A.f[id]));
+// the cells one (ida) and two (idb) steps inward of edge cell (x, y)
+__device__ __forceinline__ void edge_inward(int nx, int ny, int x, int y, int &ida, int &idb) {
   const bool top = y == 0, bottom = y == ny - 1, left = x == 0, right = x == nx - 1;
   const int ja = top ? y + 1 : bottom ? y - 1 : y;
   const int jb = top ? y + 2 : bottom ? y - 2 : y;
   const int ia = left ? x + 1 : right ? x - 1 : x;
   const int ib = left ? x + 2 : right ? x - 2 : x;
-  const int ida = ja * nx + ia, idb = jb * nx + ib;
+  ida = ja * nx + ia;
+  idb = jb * nx + ib;
+}
+
+template <typename T>
+__device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T *__restrict__ um,
+                          int x, int y) {
+  const int id = y * A.nx + x;
+  const T r = A.c0dt_h * sqrt(T(1) + T(A.f[id]));
+  int ida, idb;
+  edge_inward(A.nx, A.ny, x, y, ida, idb);
   return mur_edge<T>(r, u0[id], u0[ida], u0[idb], um[id], um[ida]);
 }
 
@@ -722,6 +728,44 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
     fp = fc;
     fc = nf;
     e1 = ne;
+  }
+}
+
+// The edge pass after forward2_kernel: rows 0 and ny-1 of level k+2 and,
+// when the next launch is also a pair, of level k+3 (its one-way rule reads
+// the cell's own level-k+2 value, formed here by the same thread, and
+// interior cells already stored), then the receiver records of levels k+1
+// and k+2.  Replaces three edge launches per pair with one.
+__global__ __launch_bounds__(256) void forward2_edge_kernel(Fwd2EdgeArgs A) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.nring) return;
+  const int shot = blockIdx.y, gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
+  const int2 c = A.ring[i];
+  const int nx = A.nx;
+  if (c.y < 0) {
+    const int x = c.x % nx, y = c.x / nx;
+    const float r = A.c0dt_h * sqrtf(1.f + A.f[c.x]);
+    int ida, idb;
+    edge_inward(nx, A.ny, x, y, ida, idb);
+    const float *__restrict__ u0 = A.u0 + so;
+    const float *__restrict__ u1 = A.up1 + so;
+    const float v2 = mur_edge<float>(r, u1[c.x], u1[ida], u1[idb], u0[c.x], u0[ida]);
+    A.up2[so + c.x] = v2;
+    if (A.up3) {
+      const float *__restrict__ u2 = A.up2 + so;
+      A.up3[so + c.x] = mur_edge<float>(r, v2, u2[ida], u2[idb], u1[c.x], u1[ida]);
+    }
+  } else {
+    const long long o1 = ((long long)gs * A.nt + A.level1) * A.nr + c.y, o2 = o1 + A.nr;
+    if (A.rec1) {
+      const float v = A.up1[so + c.x];
+      A.rec[o1] = A.meas ? A.meas[o1] - v : v;
+    }
+    if (A.rec2) {
+      const float v = A.up2[so + c.x];
+      A.rec[o2] = A.meas ? A.meas[o2] - v : v;
+    }
   }
 }
 
@@ -1754,6 +1798,12 @@ hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st) {
     return hipErrorInvalidValue;
   }
 #undef TOMO_FWD2
+  return hipGetLastError();
+}
+
+hipError_t launch_forward2_edges(const Fwd2EdgeArgs &a, int nshots, hipStream_t st) {
+  if (a.nring <= 0) return hipSuccess;
+  forward2_edge_kernel<<<dim3((a.nring + 255) / 256, nshots, 1), 256, 0, st>>>(a);
   return hipGetLastError();
 }
 

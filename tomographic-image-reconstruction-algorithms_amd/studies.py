@@ -13,7 +13,9 @@
   sets the target (its final relative error); the learned run takes 3
   learned iterations (the network of ``weights/preconditioner.npy``, with
   its misfit-checked plain fallback) and continues with plain iterations
-  until it reaches the target.
+  until it reaches the target.  A phantom on which plain SIRT ends with a
+  larger error than it started from has no meaningful target; its run is
+  reported with ``plain_progress`` false and left out of the speedups.
 - ``train``: the training run that produced ``weights/preconditioner.npy``:
   16 seeded 256^2 phantoms (seeds 7000-7015; 64 transducers, 2048 steps,
   checkpoint 200), 40 plain-SIRT iterations each at relaxation 0.1 x the
@@ -143,4 +145,8 @@ def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = 0.1, 
         "learned_iterations": None if reached is None else reached + 1,
         "learned_eps": [round(v, 3) for v in eps], "learned_steps_accepted": acc[:learned_iters],
         "speedup": None if t_learned is None else round(tp.seconds[-1] / t_learned, 2),
+        # the target is meaningful only where plain SIRT itself lowered the
+        # error over its 50 iterations
+        "plain_progress": bool(tp.eps[-1] < tp.eps[0]),
+        "plain_best_eps": round(min(tp.eps), 4),
     }
