@@ -13,12 +13,12 @@ checkpointed every 200 steps, smooth random velocity (Gaussian-filtered noise,
 sigma 8 cells, +-8 %) plus 4 sharp inclusions; measured traces from the FP64
 solver.
 
-Also reported: the roofline of the forward stencil kernel (algorithmic 16 B
-per cell-update, average launch time from HIP events on the engine's
-stream; next to it the bytes this engine's layout moves, and the measured
-HBM bytes per launch from the committed counter summary), the adjoint
-kernel's figures (20 B), and the CPU baseline (the oracle's FP32
-transcription, OpenMP over shots) on a bounded sample of the same workload.
+Also reported: the roofline of the dominant stencil kernel (the bytes per
+cell-update the algorithm must move under its blocking, the average launch
+time from HIP events on the engine's stream, and the measured HBM bytes per
+launch from the committed counter summary), both stencil kernels' figures,
+and the CPU baseline (the oracle's FP32 transcription, OpenMP over shots) on
+a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -39,23 +39,24 @@ from _pkg import tomo  # noqa: E402
 import tomo_amd.workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per cell-update of one shot, the per-unit figures of
-# the roofline: a forward cell-update reads u(k), u(k-1) and the per-cell
-# coefficient and writes u(k+1) (16 B); an adjoint cell-update fused with the
-# correlation reads z(k+1), z(k+2), the coefficient and the recomputed u(k)
-# and writes z(k), with the gradient accumulated in registers (20 B).
-FWD_BYTES = 16.0
-ADJ_BYTES = 20.0
-# What this engine's kernels move per cell-update by their own data layout:
-# the coefficient image is shared by every shot of a launch (4/shots B,
-# L2-resident); the two-step forward kernel reads u(k), u(k-1) and writes
-# u(k+1), u(k+2) once per two steps (8 B), the one-step kernel 12 B; the
-# two-step adjoint reads z(k+1), z(k+2), u(k), u(k-1) and the per-shot
-# gradient and writes z(k), z(k-1) and the gradient once per two steps
-# (16 B), the one-step adjoint re-reads and re-writes the gradient each step
-# (24 B).
-FWD_LAYOUT_BYTES = {1: 12.0, 2: 8.0}
-ADJ_LAYOUT_BYTES = {1: 24.0, 2: 16.0}
+# HBM bytes per cell-update of one shot that the stencil algorithm must move
+# (roofline.achieved): the coefficient image is shared by every shot of a
+# launch (4/shots B, L2-resident); the two-step forward kernel reads u(k),
+# u(k-1) and writes u(k+1), u(k+2) once per two steps (8 B), the one-step
+# kernel 12 B; the two-step adjoint reads z(k+1), z(k+2), u(k), u(k-1) and
+# the per-shot gradient and writes z(k), z(k-1) and the gradient once per two
+# steps (16 B), the one-step adjoint re-reads and re-writes the gradient each
+# step (24 B).
+FWD_BYTES = {1: 12.0, 2: 8.0}
+ADJ_BYTES = {1: 24.0, 2: 16.0}
+# The per-unit figures of the one-step formulation, reported beside them: a
+# forward cell-update reads u(k), u(k-1) and the per-cell coefficient and
+# writes u(k+1) (16 B); an adjoint cell-update fused with the correlation
+# reads z(k+1), z(k+2), the coefficient and u(k) and writes z(k), with the
+# gradient in registers (20 B).  With two steps per launch these exceed what
+# the kernels must move.
+FWD_UNIT_BYTES = 16.0
+ADJ_UNIT_BYTES = 20.0
 # Measured HBM bytes per launch of the stencil kernels (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE of a --counter-pass run, scaled by the counters'
 # reading for a 1 GiB device copy).
@@ -249,10 +250,10 @@ def main():
     adj_cells_launch = shots * n * n * ab
     fwd_avg_ms = tm["fwd_ms"] / max(tm["fwd_launches"], 1) * tb
     adj_avg_ms = tm["adj_ms"] / max(tm["adj_launches"], 1) * ab
-    fwd_gbs = FWD_BYTES * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
-    adj_gbs = ADJ_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
-    fwd_layout_gbs = FWD_LAYOUT_BYTES[tb] * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
-    adj_layout_gbs = ADJ_LAYOUT_BYTES[ab] * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
+    fwd_gbs = FWD_BYTES[tb] * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
+    adj_gbs = ADJ_BYTES[ab] * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
+    fwd_unit_gbs = FWD_UNIT_BYTES * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
+    adj_unit_gbs = ADJ_UNIT_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
     fwd_kernel = "forward2_kernel" if tb == 2 else "forward_vec_kernel"
     adj_kernel = "adjoint2_kernel" if ab == 2 else "adjoint_vec_kernel"
     fwd_traffic = load_traffic(fwd_kernel, shape, tb) if args.workload == "c3" else None
@@ -283,18 +284,16 @@ def main():
         "forward": {
             "kernel": f"{fwd_kernel} (with its edge/receiver passes)",
             "achieved": round(fwd_gbs, 1), "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": fwd_traffic,
-            "bytes_per_cell_update": FWD_BYTES, "cells_per_launch": fwd_cells_launch,
+            "bytes_per_cell_update": FWD_BYTES[tb], "cells_per_launch": fwd_cells_launch,
             "avg_launch_ms": round(fwd_avg_ms, 5), "total_ms": round(tm["fwd_ms"], 2),
-            "layout_bytes_per_cell_update": FWD_LAYOUT_BYTES[tb], "layout_achieved": round(fwd_layout_gbs, 1),
-            "layout_frac": round(fwd_layout_gbs / HBM_PEAK_GBS, 4),
+            "unit_bytes_per_cell_update": FWD_UNIT_BYTES, "unit_achieved": round(fwd_unit_gbs, 1),
         },
         "adjoint": {
             "kernel": f"{adj_kernel} (with its edge/receiver pass)",
             "achieved": round(adj_gbs, 1), "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "traffic": adj_traffic,
-            "bytes_per_cell_update": ADJ_BYTES, "cells_per_launch": adj_cells_launch,
+            "bytes_per_cell_update": ADJ_BYTES[ab], "cells_per_launch": adj_cells_launch,
             "avg_launch_ms": round(adj_avg_ms, 5), "total_ms": round(tm["adj_ms"], 2),
-            "layout_bytes_per_cell_update": ADJ_LAYOUT_BYTES[ab], "layout_achieved": round(adj_layout_gbs, 1),
-            "layout_frac": round(adj_layout_gbs / HBM_PEAK_GBS, 4),
+            "unit_bytes_per_cell_update": ADJ_UNIT_BYTES, "unit_achieved": round(adj_unit_gbs, 1),
         },
         "cpu_baseline": None,
     }

@@ -1,6 +1,8 @@
 """The command-line drivers that mirror the reference executables: arguments,
 file names, text layout and stop rules (generate-field.cu, generate-data.cu,
-sirt-mimo-io.cu, sirt-mimo-norm-mem.cu, art-mimo-io.cu)."""
+sirt-mimo-io.cu, sirt-mimo-norm-mem.cu, art-mimo-io.cu, sirt-mimo-seq.cu,
+sirt-test-weight.cu, and the averaging / group-serial variants
+sirt-mimo-avg.cu, sirt-mimo-weight.cu, sirt-iter.cu, sirt-iter-norm.cu)."""
 import os
 import subprocess
 
@@ -33,6 +35,65 @@ def test_usage_and_filename_errors(tmp_path):
     assert r.returncode == 1 and "Usage" in r.stderr
     r = run(["sirt-mimo-io", "a.b.txt", 10, 1, 1, 1, 1, 1, 1], tmp_path)
     assert r.returncode == 1 and "only one period" in r.stderr
+    # the variants' argument lists (sirt-mimo-avg.cu:113-115,
+    # sirt-mimo-weight.cu:68-70, sirt-iter.cu:410-412, sirt-iter-norm.cu:464-466)
+    for exe, extra in (("sirt-mimo-avg", ""), ("sirt-mimo-weight", " <alpha> <beta> <omega>"),
+                       ("sirt-iter", " <omega> <alpha>"), ("sirt-iter-norm", " <omega> <alpha>")):
+        r = run([exe, "fo.txt", 10], tmp_path)
+        assert r.returncode == 1
+        assert r.stderr.startswith("Usage: ") and r.stderr.rstrip().endswith(
+            "<fo_filename> <sensor group size> <target epsilon> <max iterations>" + extra), r.stderr
+    r = run(["sirt-iter", "a.b.txt", 10, 1, 1, 1, 1], tmp_path)
+    assert r.returncode == 1 and "only one period" in r.stderr
+
+
+def table_eps(stdout):
+    return [float(l.split()[1]) for l in stdout.splitlines()[2:] if l.strip()]
+
+
+@pytest.mark.gpu
+def test_averaging_and_group_serial_variants(tmp_path):
+    """Each variant's epsilon trajectory equals the engine's update it maps
+    to, run through the binding on the driver's own data files."""
+    print("[case] SIRT variants G=160", flush=True)
+    assert run(["generate-field"], tmp_path).returncode == 0
+    assert run(["generate-data", "fo.txt", 160], tmp_path).returncode == 0
+    with tomo.Engine(tomo.config_reference()) as e:
+        ii, jj = tomo.transducers_reference()
+        e.set_transducers(ii, jj, one_based=True)
+        ng = e.set_groups(160)
+        sides = [tomo.read_text(tmp_path / f"fo-data-{s}-160.txt", (192, 401, ng)).reshape(ng, 401, 192)
+                 for s in ("bottom", "right", "top", "left")]
+        e.set_measured(e.sides_to_traces(*sides))
+        e.set_truth(tomo.read_text(tmp_path / "fo.txt", (192, 192)))
+        zero = np.zeros(192 * 192, np.float32)
+
+        def engine_eps(scales, weighted):
+            e.set_field(zero)
+            return [e.sirt_iteration(s, weighted=weighted, mode=tomo.UPDATE_MEAN)[1] for s in scales]
+
+        # averaged intermediate fields at 20000: "Iter: k / epsilon = v" lines
+        r = run(["sirt-mimo-avg", "fo.txt", 160, 0, 2], tmp_path)
+        assert r.returncode == 0, r.stderr
+        got = [float(l.split("=")[1]) for l in r.stdout.splitlines() if l.startswith("epsilon =")]
+        np.testing.assert_allclose(got, engine_eps([20000.0, 20000.0], False), rtol=1e-6)
+        assert os.path.exists(tmp_path / "sirt-160-fo.txt")
+        # omega (alpha + beta / k), k = 1, 2
+        r = run(["sirt-mimo-weight", "fo.txt", 160, 0, 2, 1.0, 10000.0, 2.0], tmp_path)
+        assert r.returncode == 0, r.stderr
+        got = [float(l.split("=")[1]) for l in r.stdout.splitlines() if l.startswith("epsilon =")]
+        np.testing.assert_allclose(got, engine_eps([2 * (1 + 10000.0), 2 * (1 + 5000.0)], False), rtol=1e-6)
+        assert os.path.exists(tmp_path / "sirt_weight_convergence.txt")
+        # group-serial: omega (alpha + (1 - alpha) (k+1))
+        r = run(["sirt-iter", "fo.txt", 160, 0, 2, 10000.0, 0.5], tmp_path)
+        assert r.returncode == 0, r.stderr
+        np.testing.assert_allclose(table_eps(r.stdout), engine_eps([10000.0, 15000.0], False), rtol=1e-6)
+        assert os.path.exists(tmp_path / "art-160-fo.txt")
+        # group-serial weighted: omega (alpha + (1 - alpha) / (k+1)^1.5)
+        r = run(["sirt-iter-norm", "fo.txt", 160, 0, 2, 20000.0, 0.5], tmp_path)
+        assert r.returncode == 0, r.stderr
+        np.testing.assert_allclose(table_eps(r.stdout),
+                                   engine_eps([20000.0, 20000.0 * (0.5 + 0.5 / 2 ** 1.5)], True), rtol=1e-6)
 
 
 @pytest.mark.gpu
