@@ -189,11 +189,12 @@ def main():
             out = S.train_study(args.out, log=log)
         else:
             runs = [S.ttt_study(seed=int(v), log=log) for v in args.seeds.split(",")]
-            valid = [r for r in runs if r["plain_progress"]]
+            sp = [r["speedup"] for r in runs if r["speedup"] is not None]
             out = {"study": "ttt", "runs": runs,
-                   "speedups": [r["speedup"] for r in valid],
-                   "seeds_without_plain_progress": [r["phantom_seed"] for r in runs if not r["plain_progress"]],
-                   "all_reached": all(r["learned_seconds"] is not None for r in valid)}
+                   "speedups": [r["speedup"] for r in runs],
+                   "speedups_fixed50": [r["speedup_fixed50"] for r in runs if r["speedup_fixed50"] is not None],
+                   "median_speedup": float(np.median(sp)) if sp else None,
+                   "all_reached": all(r["learned_seconds"] is not None for r in runs)}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "c4":

@@ -347,10 +347,13 @@ int tomo_get_adjoint_blocking(const tomo_ctx *ctx);
  * (adjoint depth <= 2) on the context's own buffers, picks per lane width
  * the fastest forward and adjoint settings, and keeps the lane width with
  * the lowest gradient-pass cost (forward time x forward launches per
- * adjoint launch + adjoint time).  The result is set as the context's
- * shape and cached per size class for the process (force = 1 re-times).
- * out5 (may be NULL) = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth};
- * ms2 (may be NULL) = {forward, adjoint} ms per launch.  Invalidates stored
+ * adjoint launch + adjoint time).  Where two steps per launch apply
+ * (tomo_set_time_blocking), the strip heights are then re-timed on the
+ * two-step kernels, which run all but the odd tails.  The result is set as
+ * the context's shape and cached per size class for the process (force = 1
+ * re-times).  out5 (may be NULL) = {vec, fwd_rows, adj_rows, fwd_depth,
+ * adj_depth}; ms2 (may be NULL) = {forward, adjoint} ms per launch of one
+ * step (per level for the two-step kernels).  Invalidates stored
  * gradients.  No reference counterpart (the reference launches one thread
  * per cell in fixed 2-D blocks, sirt-mimo.cu:290-300). */
 int tomo_autotune(tomo_ctx *ctx, int force, int *out5, double *ms2);

@@ -435,7 +435,8 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
   b.sponge_w = a.sponge_w;
   b.sp_a = a.sp_a;
   b.sp_ab = a.sp_ab;
-  HIPCHK(c, launch_forward2(b, P, stencil_shape(c).fwd_rows, c->stream));
+  const StencilShape sh = stencil_shape(c);
+  HIPCHK(c, launch_forward2(b, P, sh.fwd_rows, sh.fwd_depth, c->stream));
   Fwd2EdgeArgs e{};
   e.u0 = u0;
   e.up1 = up1;
@@ -1500,6 +1501,53 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
       best.shape.adj_depth = bda;
       best.fwd_ms = bf;
       best.adj_ms = ba;
+    }
+  }
+  // with two steps per launch the one-step kernels only run odd tails: the
+  // strip heights are then those of the two-step kernels (same lane width)
+  if (best.shape.vec != 0) {
+    c->shape_req = best.shape;
+    if (two_step(c)) {
+      float *Z3 = c->d_z + 3 * lvl;
+      const bool adj2 = two_step_adjoint(c);
+      double bf = 1e30, ba = 1e30;
+      int brf = 0, bra = 0;
+      for (int rows : {8, 16, 32}) {
+        if (rows > c->ny) continue;
+        c->shape_req.fwd_rows = rows;
+        c->shape_req.adj_rows = rows;
+        double tf = 0.0, ta = 1e30;
+        rc = time_launches(c, reps, [&]() -> int {
+          return forward_step2(c, fa, Z0, Z1, Z2, Z3, mid, P, false, nullptr, nullptr, true, nullptr);
+        }, &tf);
+        if (!rc && adj2) {
+          Adj2Args b = adj2_args(c, 0);
+          b.z1 = Z1;
+          b.z2 = Z2;
+          b.z0 = Z0;
+          b.zm = Z3;
+          b.u0 = pass1_level(c, P, 0, true);
+          b.um = pass1_level(c, P, 1, true);
+          b.level = mid;
+          rc = time_launches(c, reps, [&]() -> int {
+            HIPCHK(c, launch_adjoint2(b, P, adjoint2_rows(c), c->stream));
+            HIPCHK(c, launch_adjoint2_edges(b, P, c->stream));
+            return TOMO_OK;
+          }, &ta);
+        }
+        if (rc) {
+          c->shape_req = keep;
+          return rc;
+        }
+        if (tf < bf) { bf = tf; brf = rows; }
+        if (ta < ba) { ba = ta; bra = rows; }
+      }
+      best.shape.fwd_rows = brf;
+      best.fwd_ms = bf / 2;  // per level
+      if (adj2) {
+        best.shape.adj_rows = bra;
+        best.adj_ms = ba / 2;
+      }
     }
   }
   c->shape_req = keep;

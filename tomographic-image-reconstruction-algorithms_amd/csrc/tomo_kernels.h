@@ -130,7 +130,8 @@ struct Fwd2Args {
   const float *sp_a, *sp_ab;
 };
 bool forward2_supported(int nx, int sponge_w);
-hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st);
+// depth: rows whose loads are in flight ahead of the row being formed (1, 2)
+hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st);
 
 // The edge pass after launch_forward2 over ring[0 .. nring) (the top row,
 // the bottom row, then the receivers): rows 0 and ny-1 of level k+2 (and of

@@ -663,7 +663,7 @@ __device__ __forceinline__ Row<VEC> step_row(const Fwd2Args &A, const LaneSponge
   return out;
 }
 
-template <int VEC, int RY>
+template <int VEC, int RY, int PD>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) {
   const int nx = A.nx, ny = A.ny;
   __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
@@ -691,43 +691,71 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
     return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
   };
   const int r0 = max(y0 - 1, 0), r1 = min(yend, ny - 1);
-  Row<VEC> cu = ld(u0, r0 - 1), cc = ld(u0, r0), cn = ld(u0, r0 + 1);
-  Row<VEC> mc = ld(um, r0), pc = ld(pw, r0);
-  float fc = edge_f<VEC>(A.f, r0, x0, nx, true);
-  // the edge rows of u(k+1) (written by the edge pass before) are read ahead
-  Row<VEC> e1 = ld(up1, r0);
-  Row<VEC> pp = pc, v2 = cc, v1 = cc;
-  float fp = fc;
+  // the edge rows of u(k+1) (formed by the edge pass before) this strip uses
+  const Row<VEC> etop = ld(up1, r0 == 0 ? 0 : -1), ebot = ld(up1, r1 == ny - 1 ? ny - 1 : -1);
+  // queues of the next PD rows: qn[j] = u(k) row base+1+j; qm / qp / qf =
+  // u(k-1), coefficients, side-edge f of row base+j
+  Row<VEC> cu = ld(u0, r0 - 1), cc = ld(u0, r0);
+  Row<VEC> qn[PD], qm[PD], qp[PD];
+  float qf[PD];
+#pragma unroll
+  for (int j = 0; j < PD; ++j) {
+    const int y = r0 + j;
+    qn[j] = ld(u0, y <= r1 ? y + 1 : -1);
+    qm[j] = ld(um, y <= r1 ? y : -1);
+    qp[j] = ld(pw, y <= r1 ? y : -1);
+    qf[j] = edge_f<VEC>(A.f, y, x0, nx, y <= r1);
+  }
+  Row<VEC> pp = cc, v2 = cc, v1 = cc;
+  float fp = 0.f;
 #pragma unroll 1
-  for (int r = r0; r <= r1; ++r) {
-    // loads of the next row are in flight while this row computes
-    const Row<VEC> nn = ld(u0, r + 2), nm = ld(um, r + 1), np = ld(pw, r + 1);
-    const float nf = edge_f<VEC>(A.f, r + 1, x0, nx, r + 1 < ny);
-    const bool edge_next = r + 1 == ny - 1;
-    const Row<VEC> ne = ld(up1, edge_next ? r + 1 : -1);
-    Row<VEC> v0;
-    if (r == 0 || r == ny - 1) {
-      v0 = e1;  // edge row, from the edge pass
-    } else {
-      v0 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, mc, pc, fc, r, x0, A.amp1, bb);
-      if (r >= y0 && r < yend) store_row<VEC>(up1 + r * nx + x0, v0);
+  for (int base = r0; base <= r1; base += PD) {
+    // the next PD rows' loads are in flight while these PD rows compute
+    Row<VEC> nn[PD], nm[PD], np[PD];
+    float nf[PD];
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      // rows read: u(k) r0-1 .. r1+1, u(k-1) and coefficients r0 .. r1
+      const int y = base + PD + j;
+      nn[j] = ld(u0, y <= r1 ? y + 1 : -1);
+      nm[j] = ld(um, y <= r1 ? y : -1);
+      np[j] = ld(pw, y <= r1 ? y : -1);
+      nf[j] = edge_f<VEC>(A.f, y, x0, nx, y <= r1);
     }
-    const int q = r - 1;
-    if (q >= y0 && q >= 1 && q <= ny - 2) {
-      const Row<VEC> w = step_row<VEC>(A, ls, s_spa, s_spab, v2, v1, v0, cu, pp, fp, q, x0, A.amp2, bb);
-      store_row<VEC>(up2 + q * nx + x0, w);
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      const int r = base + j;
+      if (r <= r1) {  // wavefront-uniform
+        const Row<VEC> &cn = qn[j];
+        Row<VEC> v0;
+        if (r == 0) {
+          v0 = etop;
+        } else if (r == ny - 1) {
+          v0 = ebot;
+        } else {
+          v0 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, qm[j], qp[j], qf[j], r, x0, A.amp1, bb);
+          if (r >= y0 && r < yend) store_row<VEC>(up1 + r * nx + x0, v0);
+        }
+        const int q = r - 1;
+        if (q >= y0 && q >= 1 && q <= ny - 2) {
+          const Row<VEC> w = step_row<VEC>(A, ls, s_spa, s_spab, v2, v1, v0, cu, pp, fp, q, x0, A.amp2, bb);
+          store_row<VEC>(up2 + q * nx + x0, w);
+        }
+        v2 = v1;
+        v1 = v0;
+        pp = qp[j];
+        fp = qf[j];
+        cu = cc;
+        cc = cn;
+      }
     }
-    v2 = v1;
-    v1 = v0;
-    pp = pc;
-    cu = cc;
-    cc = cn;
-    cn = nn;
-    mc = nm;
-    pc = np;
-    fp = fc;
-    fc = nf;
-    e1 = ne;
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      qn[j] = nn[j];
+      qm[j] = nm[j];
+      qp[j] = np[j];
+      qf[j] = nf[j];
+    }
   }
 }
 
@@ -1776,17 +1804,20 @@ bool forward2_supported(int nx, int sponge_w) {
   return sponge_w > 0 && (nx == 128 || nx == 256 || nx == 512);
 }
 
-hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st) {
+hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st) {
   const int vec = a.nx / kBlockX;
   const int ry = (rows == 8 || rows == 32) ? rows : 16;
   const int strips = (a.ny + ry - 1) / ry;
   a.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
   const dim3 grid(a.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
+#define TOMO_FWD2_PD(V, R)                                                            \
+  if (depth >= 2) forward2_kernel<V, R, 2><<<grid, block, 0, st>>>(a);                \
+  else forward2_kernel<V, R, 1><<<grid, block, 0, st>>>(a);
 #define TOMO_FWD2(V)                                                                  \
   switch (ry) {                                                                       \
-    case 8: forward2_kernel<V, 8><<<grid, block, 0, st>>>(a); break;                  \
-    case 32: forward2_kernel<V, 32><<<grid, block, 0, st>>>(a); break;                \
-    default: forward2_kernel<V, 16><<<grid, block, 0, st>>>(a); break;                \
+    case 8: TOMO_FWD2_PD(V, 8) break;                                                 \
+    case 32: TOMO_FWD2_PD(V, 32) break;                                               \
+    default: TOMO_FWD2_PD(V, 16) break;                                               \
   }
   if (vec == 8) {
     TOMO_FWD2(8)
@@ -1798,6 +1829,7 @@ hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, hipStream_t st) {
     return hipErrorInvalidValue;
   }
 #undef TOMO_FWD2
+#undef TOMO_FWD2_PD
   return hipGetLastError();
 }
 

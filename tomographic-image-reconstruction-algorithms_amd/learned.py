@@ -158,14 +158,19 @@ class Trace:
         return None
 
 
-def run_plain(p: Problem, scale: float, iters: int, weighted: bool = False, target: float | None = None) -> Trace:
+def run_plain(p: Problem, scale: float, iters: int, weighted: bool = False, target: float | None = None,
+              stop_on_rise: bool = False) -> Trace:
+    """Plain SIRT for up to `iters` iterations; stops early at `target`, or
+    (stop_on_rise) at the first iteration whose error exceeds the previous
+    one, as the reference loop does (sirt-mimo-io.cu:379-393)."""
     tr = Trace()
     e = p.engine
     t0 = time.perf_counter()
     for _ in range(iters):
         m, eps = e.sirt_iteration(scale, weighted=weighted, mode=UPDATE_MEAN)
+        rose = stop_on_rise and len(tr.eps) > 0 and eps > tr.eps[-1]
         tr.add(m, eps, True, time.perf_counter() - t0)
-        if target is not None and eps <= target:
+        if (target is not None and eps <= target) or rose:
             break
     return tr
 
