@@ -1040,22 +1040,33 @@ __global__ __launch_bounds__(256) void adjoint_ring_kernel(Launch<AdjArgs> L) {
   if (COV) A.cov[gs * n2 + c.x] = fused(uval, uval, A.cov[gs * n2 + c.x]);
 }
 
+// Sponge factors of cell (x, y) from an interleaved LDS table {a, ab}: the
+// same table entries as lane_sponge / sponge_at, looked up per cell instead
+// of held in registers (for kernels short of registers).
+__device__ __forceinline__ void sponge_cell(int x, int nx, int y, int ny, int w, const float2 *tab, float &a,
+                                            float &ab) {
+  const int d = min(min(x, nx - 1 - x), min(y, ny - 1 - y));
+  const float2 v = d < w ? tab[d] : make_float2(1.f, 1.f);
+  a = v.x;
+  ab = v.y;
+}
+
 // One adjoint row by the interior rule, for full-row waves (the x-neighbours
 // of every lane come by DPP; the side-edge cells get the rule's value and are
 // overwritten by the caller).  pn / zn, ps / zs = coefficient and z(k+1)
 // rows above and below, pc / zc the row itself, z2 = z(k+2) at the row.
 template <int VEC>
-__device__ __forceinline__ Row<VEC> adjoint_row(const LaneSponge<VEC> &ls, const float *s_spa, const float *s_spab,
-                                                const Row<VEC> &pn, const Row<VEC> &zn, const Row<VEC> &ps,
-                                                const Row<VEC> &zs, const Row<VEC> &pc, const Row<VEC> &zc,
-                                                const Row<VEC> &z2, int y, int ny, int w) {
+__device__ __forceinline__ Row<VEC> adjoint_row(const float2 *s_sp, int x0, int nx, const Row<VEC> &pn,
+                                                const Row<VEC> &zn, const Row<VEC> &ps, const Row<VEC> &zs,
+                                                const Row<VEC> &pc, const Row<VEC> &zc, const Row<VEC> &z2, int y,
+                                                int ny, int w) {
   const float zl = from_prev_lane(zc.e[VEC - 1]), zr = from_next_lane(zc.e[0]);
   const float pl = from_prev_lane(pc.e[VEC - 1]), pr = from_next_lane(pc.e[0]);
   Row<VEC> out;
 #pragma unroll
   for (int e = 0; e < VEC; ++e) {
     float a, ab;
-    sponge_at<VEC>(ls, e, y, ny, w, s_spa, s_spab, a, ab);
+    sponge_cell(x0 + e, nx, y, ny, w, s_sp, a, ab);
     const float zle = e == 0 ? zl : zc.e[e - 1];
     const float zre = e == VEC - 1 ? zr : zc.e[e + 1];
     const float ple = e == 0 ? pl : pc.e[e - 1];
@@ -1114,11 +1125,13 @@ __device__ __forceinline__ void side_copies(Row<VEC> &z, int x0, int nx) {
 template <int VEC, int RY>
 __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) {
   const int nx = A.nx, ny = A.ny;
-  __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
-  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
-    s_spa[i] = A.sp_a[i];
-    s_spab[i] = A.sp_ab[i];
-  }
+  __shared__ float2 s_sp[kSpongeLds];
+  // the gradient row after level k, held from one row to the next for level
+  // k-1 in a lane-private LDS slot instead of registers ([wave][cell][lane]:
+  // conflict-free)
+  __shared__ float s_acc[kBlockY][VEC][kBlockX];
+  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY)
+    s_sp[i] = make_float2(A.sp_a[i], A.sp_ab[i]);
   __syncthreads();
   const int lb = xcd_logical_block();
   const int shot = lb / A.blocks_per_shot;
@@ -1140,7 +1153,6 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
   const float *__restrict__ pw = A.pw;
   const float *__restrict__ rrk = A.rr + ((long long)gs * A.nt + A.level) * A.nr;  // residuals at level k
   const float *__restrict__ rrm = rrk - A.nr;                                        // and at level k-1
-  const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
   auto ld = [&](const float *__restrict__ base, int y, bool ok) {
     ok = ok && y >= 0 && y < ny;
     return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
@@ -1164,8 +1176,9 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
   Row<VEC> uu = ld(u0, rs - 1, lrow(rs - 1)), uc = ld(u0, rs, lrow(rs)), un = ld(u0, rs + 1, lrow(rs + 1));
   Row<VEC> m2 = ld(um, rs - 2, false), m1 = ld(um, rs - 1, lrow(rs - 1)), m0 = ld(um, rs, lrow(rs));
   Row<VEC> zz = ld(z2, rs, true);
-  Row<VEC> ac = ld(acc, rs, own(rs)), aq = ld(acc, 0, false);
-  Row<VEC> v2 = aq, v1 = aq;  // z(k) rows r-2, r-1
+  Row<VEC> ac = ld(acc, rs, own(rs));
+  Row<VEC> v2 = ld(acc, 0, false), v1 = v2;  // z(k) rows r-2, r-1
+  float(&stash)[VEC][kBlockX] = s_acc[threadIdx.y];
   int dr = desc(rs), dn = desc(rs + 1), dq = -1;
   float rk = resid(dr, rrk), rkm = resid(dr, rrm), rq = 0.f;
 
@@ -1182,7 +1195,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
     // z(k) at row r
     Row<VEC> v0;
     if (r <= re) {
-      v0 = adjoint_row<VEC>(ls, s_spa, s_spab, pu, zu, pn, zn, pc, zc, zz, r, ny, A.sponge_w);
+      v0 = adjoint_row<VEC>(s_sp, x0, nx, pu, zu, pn, zn, pc, zc, zz, r, ny, A.sponge_w);
       inject<VEC>(v0, dr, rk, A.gain);
       if (own(r)) {
         ac = correlate_row<VEC>(uu, uc, un, ac, v0, x0, nx, r, ny, A.inv_h2);
@@ -1197,7 +1210,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
     // z(k-1) at row q = r-1
     const int q = r - 1;
     if (own(q)) {
-      Row<VEC> w = adjoint_row<VEC>(ls, s_spa, s_spab, pp, v2, pc, v0, pu, v1, zu, q, ny, A.sponge_w);
+      Row<VEC> aq;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) aq.e[e] = stash[e][threadIdx.x];
+      Row<VEC> w = adjoint_row<VEC>(s_sp, x0, nx, pp, v2, pc, v0, pu, v1, zu, q, ny, A.sponge_w);
       inject<VEC>(w, dq, rq, A.gain);
       const Row<VEC> ao = correlate_row<VEC>(m2, m1, m0, aq, w, x0, nx, q, ny, A.inv_h2);
       store_row<VEC>(zm + q * nx + x0, w);
@@ -1205,7 +1221,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
     }
     // residuals of row r+1 (its descriptor was read a row ago)
     const float rkn = resid(dn, rrk), rkmn = resid(dn, rrm);
-    aq = ac;
+    if (own(r)) {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) stash[e][threadIdx.x] = ac.e[e];
+    }
     ac = an;
     v2 = v1;
     v1 = v0;
