@@ -157,6 +157,9 @@ def main():
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
     ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
     ap.add_argument("--out", default="gpurun_out/preconditioner.npy", help="parameters written by --study train")
+    ap.add_argument("--phantoms", type=int, default=16, help="training phantoms of --study train")
+    ap.add_argument("--train-steps", type=int, default=1500, help="Adam steps of --study train")
+    ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--time-blocking", type=int, default=2, choices=[1, 2],
                     help="forward steps per launch (2: two steps with the intermediate level in registers)")
@@ -186,9 +189,9 @@ def main():
         if args.study == "c2":
             out = S.c2_study(log=log)
         elif args.study == "train":
-            out = S.train_study(args.out, log=log)
+            out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps, log=log)
         else:
-            runs = [S.ttt_study(seed=int(v), log=log) for v in args.seeds.split(",")]
+            runs = [S.ttt_study(seed=int(v), weights=args.weights, log=log) for v in args.seeds.split(",")]
             sp = [r["speedup"] for r in runs if r["speedup"] is not None]
             out = {"study": "ttt", "runs": runs,
                    "speedups": [r["speedup"] for r in runs],

@@ -639,7 +639,6 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
         b.um = segment_level(c, P, m, k - 1);
         b.level = k;
         HIPCHK(c, launch_adjoint2(b, np, adjoint2_rows(c), c->stream));
-        HIPCHK(c, launch_adjoint2_edges(b, np, c->stream));
         launches += 2;
         q -= 2;
         continue;
@@ -1542,7 +1541,6 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
           b.level = mid;
           rc = time_launches(c, reps, [&]() -> int {
             HIPCHK(c, launch_adjoint2(b, P, adjoint2_rows(c), c->stream));
-            HIPCHK(c, launch_adjoint2_edges(b, P, c->stream));
             return TOMO_OK;
           }, &ta);
         }

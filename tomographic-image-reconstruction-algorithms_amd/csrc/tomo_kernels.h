@@ -158,8 +158,8 @@ hipError_t launch_forward2_edges(const Fwd2EdgeArgs &a, int nshots, hipStream_t 
 // preset, point receivers with additive injection, no coverage; a wave
 // holds full rows, nx == 64 VEC).  z(k) is kept in registers one row ahead
 // of z(k-1), the receivers are injected in registers, and the gradient row
-// is read and written once for both levels.  Rows 1..ny-2 of both levels;
-// rows 0 and ny-1 and their correlation are left to launch_adjoint2_edges.
+// is read and written once for both levels.  All rows of both levels: rows
+// 0 and ny-1 are the copies of rows 1 and ny-2, with their correlation.
 struct Adj2Args {
   const float *z1, *z2;  // adjoint levels k+1, k+2 of local shot 0
   float *z0, *zm;        // levels k, k-1 (written)
@@ -181,9 +181,6 @@ struct Adj2Args {
 // nx for which the two-step adjoint exists (vec = nx / 64)
 bool adjoint2_supported(int nx, int sponge_w);
 hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st);
-// rows 0 and ny-1 of levels k and k-1 (copies of rows 1 and ny-2) and their
-// correlation, level k first; u0 / um / z0 / zm / acc as in the main launch
-hipError_t launch_adjoint2_edges(const Adj2Args &a, int nshots, hipStream_t st);
 // the edge/receiver pass alone over a.ring[0 .. a.nring) (grid: cells x shots)
 hipError_t launch_forward_ring(const FwdArgs<float> &a, int nshots, hipStream_t st);
 

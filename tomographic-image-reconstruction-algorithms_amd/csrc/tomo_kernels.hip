@@ -1095,6 +1095,61 @@ __device__ __forceinline__ Row<VEC> correlate_row(const Row<VEC> &uu, const Row<
   return out;
 }
 
+// Lu of the top (TOP) or bottom edge row from registers, for full-row waves:
+// E = the edge row, I = the row one inward, I2 = two inward.  The same
+// operations as edge_lu: clamped 5-point Laplacian whose corner neighbours
+// read as the mean of their two edge neighbours, and at a corner cell the
+// mean of the Laplacians at its two edge neighbours (laplace,
+// laplace_corners, sirt-mimo.cu:776-831).
+template <int VEC, bool TOP>
+__device__ __forceinline__ Row<VEC> edge_row_lu(const Row<VEC> &E, const Row<VEC> &I, const Row<VEC> &I2, int x0,
+                                                int nx, float inv_h2) {
+  const float el = from_prev_lane(E.e[VEC - 1]), er = from_next_lane(E.e[0]);
+  const float cm0 = 0.5f * (I.e[0] + E.e[1]);                  // corner mean at x = 0 (lane 0)
+  const float cmr = 0.5f * (I.e[VEC - 1] + E.e[VEC - 2]);      // at x = nx-1 (last lane)
+  Row<VEC> out;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    const int x = x0 + e;
+    float w = e == 0 ? el : E.e[e - 1];
+    float ea = e == VEC - 1 ? er : E.e[e + 1];
+    w = x - 1 == 0 ? cm0 : w;
+    ea = x + 1 == nx - 1 ? cmr : ea;
+    const float n = TOP ? E.e[e] : I.e[e], s = TOP ? I.e[e] : E.e[e];
+    out.e[e] = lap5(n, s, w, ea, E.e[e], inv_h2);
+  }
+  if (x0 == 0) {
+    const float b = TOP ? lap5(cm0, I2.e[0], I.e[0], I.e[1], I.e[0], inv_h2)
+                        : lap5(I2.e[0], cm0, I.e[0], I.e[1], I.e[0], inv_h2);
+    out.e[0] = (out.e[1] + b) / 2.f;
+  }
+  if (x0 + VEC == nx) {
+    constexpr int L = VEC - 1;
+    const float b = TOP ? lap5(cmr, I2.e[L], I.e[L - 1], I.e[L], I.e[L], inv_h2)
+                        : lap5(I2.e[L], cmr, I.e[L - 1], I.e[L], I.e[L], inv_h2);
+    out.e[L] = (out.e[L - 1] + b) / 2.f;
+  }
+  return out;
+}
+
+// Row 0 (TOP) or ny-1 of one adjoint level, from row 1 / ny-2 already formed
+// (z, with its side copies) and the u rows E, I, I2 (edge row first): stores
+// the copy and adds its correlation to the edge row of the gradient.  Run by
+// the one strip that owns row 1 / ny-2; level k before level k-1, the order
+// of two one-step launches.
+template <int VEC, bool TOP>
+__device__ __forceinline__ void adjoint_edge_row(const Row<VEC> &z, const Row<VEC> &E, const Row<VEC> &I,
+                                                 const Row<VEC> &I2, float *__restrict__ zrow,
+                                                 float *__restrict__ arow, int x0, int nx, float inv_h2) {
+  store_row<VEC>(zrow + x0, z);
+  const Row<VEC> lu = edge_row_lu<VEC, TOP>(E, I, I2, x0, nx, inv_h2);
+  const Row<VEC> a = load_row<VEC>(arow + x0, true);
+  Row<VEC> out;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) out.e[e] = fused(z.e[e], lu.e[e], a.e[e]);
+  store_row<VEC>(arow + x0, out);
+}
+
 // additive injection at the lane's receiver cell (descriptor d = receiver *
 // VEC + cell, or -1), as adjoint_slow does for a point receiver
 template <int VEC>
@@ -1115,15 +1170,17 @@ __device__ __forceinline__ void side_copies(Row<VEC> &z, int x0, int nx) {
 // strip's own rows; then it forms z(k-1) at row r-1 from the z(k) window held
 // in registers (rows r-2, r-1, r) and z(k+1) row r-1, and correlates it with
 // Lu(k-1) into the same gradient row, read once and written once.  z(k) is
-// formed on one halo row above and below the strip.  Row 0 of z(k) is the
-// copy of row 1 and row ny-1 the copy of row ny-2 (backpropagation2), so the
-// window needs nothing from outside the strip's rows and halo.  Per two
+// formed on one halo row above and below the strip.  Row 0 of each level is
+// the copy of row 1 and row ny-1 the copy of row ny-2 (backpropagation2), so
+// the window needs nothing from outside the strip's rows and halo; the strip
+// that owns row 1 (ny-2) also writes row 0 (ny-1) and its correlation.  Per two
 // cell-updates: reads z(k+1), z(k+2), u(k), u(k-1), the gradient, writes z(k),
 // z(k-1), the gradient (16 B per cell-update instead of 24).  The arithmetic
 // of every cell is that of adjoint_vec_kernel and adjoint_ring_kernel, so the
 // fields and gradients agree with two one-step launches bit for bit.
 template <int VEC, int RY>
-__global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) {
+__global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_eu(2))) void adjoint2_kernel(
+    Adj2Args A) {
   const int nx = A.nx, ny = A.ny;
   __shared__ float2 s_sp[kSpongeLds];
   // the gradient row after level k, held from one row to the next for level
@@ -1200,6 +1257,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
       if (own(r)) {
         ac = correlate_row<VEC>(uu, uc, un, ac, v0, x0, nx, r, ny, A.inv_h2);
         store_row<VEC>(z0 + r * nx + x0, v0);
+        // rows 0 and ny-1 of z(k): copies of rows 1 and ny-2
+        if (r == 1) adjoint_edge_row<VEC, true>(v0, uu, uc, un, z0, acc, x0, nx, A.inv_h2);
+        if (r == ny - 2)
+          adjoint_edge_row<VEC, false>(v0, un, uc, uu, z0 + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
       } else {
         side_copies<VEC>(v0, x0, nx);
       }
@@ -1218,6 +1279,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
       const Row<VEC> ao = correlate_row<VEC>(m2, m1, m0, aq, w, x0, nx, q, ny, A.inv_h2);
       store_row<VEC>(zm + q * nx + x0, w);
       store_row<VEC>(acc + q * nx + x0, ao);
+      // rows 0 and ny-1 of z(k-1)
+      if (q == 1) adjoint_edge_row<VEC, true>(w, m2, m1, m0, zm, acc, x0, nx, A.inv_h2);
+      if (q == ny - 2)
+        adjoint_edge_row<VEC, false>(w, m0, m1, m2, zm + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
     }
     // residuals of row r+1 (its descriptor was read a row ago)
     const float rkn = resid(dn, rrk), rkmn = resid(dn, rrm);
@@ -1251,28 +1316,6 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint2_kernel(Adj2Args A) 
   }
 }
 
-// Rows 0 and ny-1 of z(k) and z(k-1) after adjoint2_kernel: copies of rows 1
-// and ny-2 (equal to adjoint_slow's inward-cell rule there, which has no
-// receivers), and their correlation with the clamped Laplacians, level k
-// first (the order of two one-step launches).
-__global__ __launch_bounds__(256) void adjoint2_edge_kernel(Adj2Args A) {
-  const int nx = A.nx, ny = A.ny;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= 2 * nx) return;
-  const int shot = blockIdx.y, gs = A.shot0 + shot;
-  const long long so = (long long)shot * A.sstride;
-  const long long n2 = (long long)nx * ny;
-  const int x = i % nx, y = i < nx ? 0 : ny - 1, yi = y == 0 ? 1 : ny - 2;
-  const int id = y * nx + x, idi = yi * nx + x;
-  float a = A.acc[gs * n2 + id];
-  const float zk = A.z0[so + idi];
-  A.z0[so + id] = zk;
-  a = fused(zk, edge_lu(A.u0 + so, nx, ny, x, y, A.inv_h2), a);
-  const float zkm = A.zm[so + idi];
-  A.zm[so + id] = zkm;
-  a = fused(zkm, edge_lu(A.um + so, nx, ny, x, y, A.inv_h2), a);
-  A.acc[gs * n2 + id] = a;
-}
 
 // ------------------------------------------------------------- elementwise
 
@@ -1882,11 +1925,6 @@ hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st) {
     return hipErrorInvalidValue;
   }
 #undef TOMO_ADJ2
-  return hipGetLastError();
-}
-
-hipError_t launch_adjoint2_edges(const Adj2Args &a, int nshots, hipStream_t st) {
-  adjoint2_edge_kernel<<<dim3((2 * a.nx + 255) / 256, nshots, 1), 256, 0, st>>>(a);
   return hipGetLastError();
 }
 

@@ -89,13 +89,14 @@ def _mask(c):
     return m.ravel()
 
 
-def train_study(out: str, log=None) -> dict:
-    """Reproduce the shipped network (see the module docstring); writes the
-    parameters (float32, numpy .npy) to `out`."""
+def train_study(out: str, phantoms: int = 16, train_steps: int = 1500, log=None) -> dict:
+    """Reproduce the shipped network (see the module docstring; `phantoms`
+    seeded phantoms from 7000); writes the parameters (float32, numpy .npy)
+    to `out`."""
     net = Preconditioner(device=0, seed=1)
     t = time.perf_counter()
-    rep = L.train(net, 256, range(7000, 7016), 40, frac=0.1, rounds=0, train_steps=1500, batch=8, lr=1e-3,
-                  target="best", log=log)
+    rep = L.train(net, 256, range(7000, 7000 + phantoms), 40, frac=0.1, rounds=0, train_steps=train_steps,
+                  batch=8, lr=1e-3, target="best", log=log)
     np.save(out, net.get_params(), allow_pickle=False)
     net.close()
     return {"study": "train", "out": out, "phantoms": rep.phantoms, "samples": rep.samples,
@@ -104,13 +105,13 @@ def train_study(out: str, log=None) -> dict:
 
 
 def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = 0.1, seed: int = 9100,
-              weights: str = WEIGHTS, log=None) -> dict:
+              weights: str | None = None, log=None) -> dict:
     """Plain SIRT (stop rule, at most `plain_iters` iterations) sets the
     target; the learned run takes `learned_iters` learned iterations and
     continues with plain ones until it reaches the target."""
     n = 512
     net = Preconditioner(device=0)
-    net.set_params(np.load(weights, allow_pickle=False))
+    net.set_params(np.load(weights or WEIGHTS, allow_pickle=False))
     p = L.Problem(n, L.phantom(n, seed), steps=4000, shots=128, checkpoint=200)
     t = time.perf_counter()
     scale = L.calibrate_scale(p, frac)
