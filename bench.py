@@ -113,10 +113,11 @@ def load_traffic(kernel, shape, tb):
 
 
 def counter_pass(args):
-    """One FP32 forward (data) and one gradient of the C3 workload, then a
-    1 GiB device copy: the program of the rocprofv3 --pmc runs that measure
-    the stencil kernels' HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE; the
-    copy's reading gives the counters' scale)."""
+    """One FP32 forward (data) and one gradient (or, --forward-only, the
+    recording forward pass of one) of the C3 workload, then a 1 GiB device
+    copy: the program of the rocprofv3 --pmc runs that measure the stencil
+    kernels' HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE; the copy's
+    reading gives the counters' scale)."""
     import torch
     log = (lambda m: print("[counters]", m, file=sys.stderr, flush=True))
     c, ii, jj, fo = make_workload("c3")
@@ -132,8 +133,12 @@ def counter_pass(args):
     meas = e.forward(fp64=False)
     e.set_field(np.zeros(fo.size, np.float32))
     e.set_measured(meas)
-    log("gradient")
-    e.gradient()
+    if args.forward_only:
+        log("misfit (recording forward pass)")
+        e.misfit()
+    else:
+        log("gradient")
+        e.gradient()
     shape = list(e.get_shape())
     tb, ab = e.time_blocking(), e.adjoint_blocking()
     e.close()
@@ -157,8 +162,8 @@ def main():
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
     ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
     ap.add_argument("--out", default="gpurun_out/preconditioner.npy", help="parameters written by --study train")
-    ap.add_argument("--phantoms", type=int, default=16, help="training phantoms of --study train")
-    ap.add_argument("--train-steps", type=int, default=1500, help="Adam steps of --study train")
+    ap.add_argument("--phantoms", type=int, default=48, help="training phantoms of --study train")
+    ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of --study train")
     ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--time-blocking", type=int, default=2, choices=[1, 2],
@@ -173,6 +178,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--counter-pass", action="store_true",
                     help="run one C3 forward and gradient plus a 1 GiB copy (the program for rocprofv3 --pmc)")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="with --counter-pass: the recording forward pass (the misfit) instead of the gradient")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "

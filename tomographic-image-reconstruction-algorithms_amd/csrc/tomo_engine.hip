@@ -283,7 +283,7 @@ int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
   const size_t lvl = (size_t)P * c->n2;
   size_t seg_levels, ckpt_levels;
   if (!need_history) {
-    seg_levels = 3;
+    seg_levels = 4;  // rolling levels of a forward pass (4 with two steps per launch)
     ckpt_levels = 0;
   } else if (full_history(c)) {
     seg_levels = (size_t)c->nt;
@@ -401,26 +401,12 @@ int forward_step(tomo_ctx *c, FwdArgs<T> a, const T *um, const T *u0, T *up, int
   return TOMO_OK;
 }
 
-// Forward steps k and k+1 (levels k-1, k -> k+1, k+2) in one main launch,
-// with the edge passes: top/bottom rows of k+1 before (unless the previous
-// pair formed them: pre = false); after, one pass for the receiver records
-// of k+1 and k+2, the top/bottom rows of k+2 and, when the next launch is a
-// pair too (up3 = its level k+3), the top/bottom rows of k+3.
+// Forward steps k and k+1 (levels k-1, k -> k+1, k+2) in one launch, edge
+// rows included; the receiver records of both levels in the same launch when
+// every receiver has its own lane segment (else one small pass after).
 int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u0, float *up1, float *up2, int k,
-                  int P, bool record, const float *meas, float *rec, bool pre, float *up3) {
+                  int P, bool record, const float *meas, float *rec) {
   const bool rec_on = record && rec != nullptr && c->nr > 0;
-  if (pre) {  // rows 0 and ny-1 of level k+1
-    a.meas = meas;
-    a.rec = rec;
-    a.ring = c->d_ring;
-    a.um = um;
-    a.u0 = u0;
-    a.up = up1;
-    a.level = k + 1;
-    a.record = 0;
-    a.nring = 2 * c->nx;
-    HIPCHK(c, launch_forward_ring(a, P, c->stream));
-  }
   Fwd2Args b{};
   b.um = um;
   b.u0 = u0;
@@ -439,29 +425,42 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
   b.sponge_w = a.sponge_w;
   b.sp_a = a.sp_a;
   b.sp_ab = a.sp_ab;
+  b.rec1 = rec_on && k + 1 >= c->cfg.record_first;
+  b.rec2 = rec_on && k + 2 >= c->cfg.record_first;
+  const bool in_kernel = (b.rec1 || b.rec2) && c->recvseg_ok;
+  if (in_kernel) {
+    b.recv_seg = c->d_recvseg;
+    b.row_has_recv = c->d_rowflag;
+    b.meas = meas;
+    b.rec = rec;
+    b.nt = c->nt;
+    b.nr = c->nr;
+    b.level1 = k + 1;
+  }
   const StencilShape sh = stencil_shape(c);
   HIPCHK(c, launch_forward2(b, P, sh.fwd_rows, sh.fwd_depth, c->stream));
-  Fwd2EdgeArgs e{};
-  e.u0 = u0;
-  e.up1 = up1;
-  e.up2 = up2;
-  e.up3 = up3;
-  e.sstride = a.sstride;
-  e.f = a.f;
-  e.ring = c->d_ring;
-  e.rec1 = rec_on && k + 1 >= c->cfg.record_first;
-  e.rec2 = rec_on && k + 2 >= c->cfg.record_first;
-  e.nring = 2 * c->nx + ((e.rec1 || e.rec2) ? c->nr : 0);
-  e.nx = c->nx;
-  e.ny = c->ny;
-  e.nt = c->nt;
-  e.nr = c->nr;
-  e.shot0 = a.shot0;
-  e.level1 = k + 1;
-  e.meas = meas;
-  e.rec = rec;
-  e.c0dt_h = a.c0dt_h;
-  HIPCHK(c, launch_forward2_edges(e, P, c->stream));
+  if ((b.rec1 || b.rec2) && !in_kernel) {  // records over the receivers
+    Fwd2EdgeArgs e{};
+    e.u0 = u0;
+    e.up1 = up1;
+    e.up2 = up2;
+    e.sstride = a.sstride;
+    e.f = a.f;
+    e.ring = c->d_ring + 2 * c->nx;
+    e.nring = c->nr;
+    e.nx = c->nx;
+    e.ny = c->ny;
+    e.nt = c->nt;
+    e.nr = c->nr;
+    e.shot0 = a.shot0;
+    e.level1 = k + 1;
+    e.rec1 = b.rec1;
+    e.rec2 = b.rec2;
+    e.meas = meas;
+    e.rec = rec;
+    e.c0dt_h = a.c0dt_h;
+    HIPCHK(c, launch_forward2_edges(e, P, c->stream));
+  }
   return TOMO_OK;
 }
 
@@ -475,7 +474,11 @@ int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const 
   for (int p0 = g0; p0 < g0 + ng; p0 += P) {
     const int np = std::min(P, g0 + ng - p0);
     HIPCHK(c, launch_fill_f32(c->d_rr + (size_t)p0 * tr, (size_t)np * tr, 0.f, c->stream));
-    auto L = [&](int k) { return ring + (size_t)(k % 3) * ring_lvl; };
+    // FP32 with two steps per launch: a ring of 4 levels (level k+2 is written
+    // while other strips still read level k-1 on their halo rows)
+    bool two = false;
+    if constexpr (std::is_same<T, float>::value) two = two_step(c);
+    auto L = [&](int k) { return ring + (size_t)(k % (two ? 4 : 3)) * ring_lvl; };
     if (std::is_same<T, float>::value) {
       HIPCHK(c, launch_fill_f32((float *)L(0), (size_t)np * c->n2, 0.f, c->stream));
       HIPCHK(c, launch_fill_f32((float *)L(1), (size_t)np * c->n2, 0.f, c->stream));
@@ -486,9 +489,19 @@ int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const 
     FwdArgs<T> a = fwd_args<T>(c, p0, pw, spa, spab);
     TimedSpan s;
     span_begin(c, s, 0);
-    for (int k = 1; k < c->nt - 1; ++k) {
-      int rc = forward_step<T>(c, a, L(k - 1), L(k), L(k + 1), k, np, true, meas, c->d_rr);
+    for (int k = 1; k < c->nt - 1;) {
+      int rc;
+      if constexpr (std::is_same<T, float>::value) {
+        if (two && k + 1 < c->nt - 1) {
+          rc = forward_step2(c, a, L(k - 1), L(k), L(k + 1), L(k + 2), k, np, true, meas, c->d_rr);
+          if (rc) return rc;
+          k += 2;
+          continue;
+        }
+      }
+      rc = forward_step<T>(c, a, L(k - 1), L(k), L(k + 1), k, np, true, meas, c->d_rr);
       if (rc) return rc;
+      ++k;
     }
     s.launches = c->nt - 2;
     s.cells = (double)(c->nt - 2) * np * c->n2;
@@ -555,15 +568,12 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
     TimedSpan s;
     span_begin(c, s, 0);
     const bool two = two_step(c);
-    bool pre = true;  // the top/bottom rows of the pair's first level are still to form
     for (int k = 1; k < nt - 1;) {
       int rc;
       if (two && k + 1 < nt - 1) {
-        const bool next = k + 3 < nt - 1;  // the next launch is a pair
         rc = forward_step2(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
                            pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true), k, np, true, c->d_meas,
-                           c->d_rr, pre, next ? pass1_level(c, P, k + 3, true) : nullptr);
-        pre = !next;
+                           c->d_rr);
         k += 2;
       } else {
         rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
@@ -598,16 +608,13 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
       TimedSpan s;
       span_begin(c, s, 0);
       const bool two = two_step(c);
-      bool pre = true;
       for (int q = 2; q < cnt;) {
         const int k = k0 + q - 1;
         int rc;
         if (two && q + 1 < cnt) {
-          const bool next = q + 3 < cnt;
           rc = forward_step2(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
                              segment_level(c, P, m, k + 1), segment_level(c, P, m, k + 2), k, np, false, nullptr,
-                             nullptr, pre, next ? segment_level(c, P, m, k + 3) : nullptr);
-          pre = !next;
+                             nullptr);
           q += 2;
         } else {
           rc = forward_step<float>(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
@@ -1525,7 +1532,7 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
           c->shape_req.fwd_depth = depth;
           double t = 0.0;
           rc = time_launches(c, reps, [&]() -> int {
-            return forward_step2(c, fa, Z0, Z1, Z2, Z3, mid, P, false, nullptr, nullptr, true, nullptr);
+            return forward_step2(c, fa, Z0, Z1, Z2, Z3, mid, P, false, nullptr, nullptr);
           }, &t);
           if (rc) break;
           if (t < tf) { tf = t; df = depth; }

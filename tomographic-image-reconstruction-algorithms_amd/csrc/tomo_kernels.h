@@ -110,10 +110,12 @@ hipError_t launch_forward_step(const FwdArgs<T> &a, int nshots, const StencilSha
 hipError_t launch_adjoint_step(const AdjArgs &a, int nshots, const StencilShape &sh, hipStream_t st);
 
 // Two forward steps k -> k+1 -> k+2 in one launch (sponge preset, waves of
-// full rows: nx == 64 vec, vec in {2, 4, 8}).  Rows 0 and ny-1 of level k+1
-// must be in up1 before (launch_forward_ring with the top/bottom cells, or
-// the previous pair's launch_forward2_edges); rows 0 and ny-1 of level k+2
-// and the receiver records are left to the edge pass after.
+// full rows: nx == 64 vec, vec in {2, 4, 8}), every row of both levels: the
+// first and last strips form rows 0 and ny-1 of level k+1 from the inputs
+// and of level k+2 from the level-k+1 rows in their registers (one-way edge
+// rule).  With recv_seg set, the receiver records of levels k+1 (rec1) and
+// k+2 (rec2) are written in the kernel too; otherwise they are left to
+// launch_forward2_edges over the receivers.
 struct Fwd2Args {
   const float *um, *u0;  // levels k-1, k of local shot 0
   float *up1, *up2;      // levels k+1, k+2
@@ -128,15 +130,23 @@ struct Fwd2Args {
   float c0dt_h;
   int sponge_w;
   const float *sp_a, *sp_ab;
+  // receiver records in the kernel (recv_seg null: none here)
+  const int *recv_seg;   // [ny][nx / vec]: receiver * vec + cell of the lane's segment, or -1
+  const unsigned char *row_has_recv;
+  const float *meas;     // measured traces [S][nt][nr] (null: record u itself)
+  float *rec;            // traces / residuals [S][nt][nr]
+  int nt, nr;
+  int level1;            // k+1
+  int rec1, rec2;        // record levels k+1, k+2
 };
 bool forward2_supported(int nx, int sponge_w);
 // depth: rows whose loads are in flight ahead of the row being formed (1, 2)
 hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st);
 
-// The edge pass after launch_forward2 over ring[0 .. nring) (the top row,
-// the bottom row, then the receivers): rows 0 and ny-1 of level k+2 (and of
-// level k+3 into up3 when up3 is set, for a following pair), the receiver
-// records of levels k+1 (rec1) and k+2 (rec2).
+// The receiver records of a pair over ring[0 .. nring) when the main kernel
+// does not write them (ring cells with a receiver index: records of levels
+// k+1 (rec1) and k+2 (rec2); ring cells without one, the top and bottom
+// rows: rows 0 and ny-1 of level k+2, and of level k+3 into up3 when set).
 struct Fwd2EdgeArgs {
   const float *u0;        // level k of local shot 0
   float *up1, *up2, *up3; // levels k+1, k+2, k+3 (up3 may be null)

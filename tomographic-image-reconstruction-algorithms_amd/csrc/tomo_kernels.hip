@@ -624,9 +624,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
 // and forms u(k+2) at row r-1 from the u(k+1) window (rows r-2, r-1, r) and
 // u(k) row r-1.  u(k) and u(k-1) are read once per strip (plus two halo rows
 // of u(k)), u(k+1) is kept in registers: 8 B of traffic per cell-update
-// instead of 12.  Rows 0 and ny-1 of u(k+1) come from the edge pass before
-// (they are read back); those of u(k+2) are left to the edge pass after.
-// The per-cell arithmetic is that of forward_vec_kernel.
+// instead of 12.  Rows 0 and ny-1 of both levels are formed by the first and
+// last strips (edge_row_step), and in recording passes the receiver records
+// are written as the rows are formed.  The per-cell arithmetic is that of
+// forward_vec_kernel and forward_ring_kernel.
 template <int VEC>
 __device__ __forceinline__ Row<VEC> step_row(const Fwd2Args &A, const LaneSponge<VEC> &ls, const float *s_spa,
                                              const float *s_spab, const Row<VEC> &up, const Row<VEC> &at,
@@ -663,7 +664,47 @@ __device__ __forceinline__ Row<VEC> step_row(const Fwd2Args &A, const LaneSponge
   return out;
 }
 
-template <int VEC, int RY, int PD>
+// One edge row (0 or ny-1) of the next level by the one-way edge rule
+// (forward_edge, sirt-mimo.cu:566-587): E, I, I2 = the current level's edge
+// row and the rows one and two inward, Em, Im = the previous level's edge row
+// and the row one inward, fr = f on the edge row.  The corner cells use the
+// diagonal neighbours (for 2-cell lanes one of them sits in the next or
+// previous lane).
+template <int VEC>
+__device__ __forceinline__ Row<VEC> edge_row_step(const Row<VEC> &E, const Row<VEC> &I, const Row<VEC> &I2,
+                                                  const Row<VEC> &Em, const Row<VEC> &Im, const Row<VEC> &fr,
+                                                  float c0dt_h, int x0, int nx) {
+  const float i2n = from_next_lane(I2.e[0]), i2p = from_prev_lane(I2.e[VEC - 1]);
+  Row<VEC> out;
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) {
+    const float r = c0dt_h * sqrtf(1.f + fr.e[e]);
+    out.e[e] = mur_edge<float>(r, E.e[e], I.e[e], I2.e[e], Em.e[e], Im.e[e]);
+  }
+  if (x0 == 0) {
+    const float r = c0dt_h * sqrtf(1.f + fr.e[0]);
+    out.e[0] = mur_edge<float>(r, E.e[0], I.e[1], VEC > 2 ? I2.e[2 % VEC] : i2n, Em.e[0], Im.e[1]);
+  }
+  if (x0 + VEC == nx) {
+    constexpr int L = VEC - 1;
+    const float r = c0dt_h * sqrtf(1.f + fr.e[L]);
+    out.e[L] = mur_edge<float>(r, E.e[L], I.e[L - 1], VEC > 2 ? I2.e[(L + VEC - 2) % VEC] : i2p, Em.e[L],
+                               Im.e[L - 1]);
+  }
+  return out;
+}
+
+// record of the lane's receiver (descriptor d = receiver * VEC + cell, or -1)
+// at one level: the residual meas - u, or u without measured traces
+template <int VEC>
+__device__ __forceinline__ void record_lane(const Row<VEC> &u, int d, float meas_v, bool has_meas,
+                                            float *__restrict__ rec_level) {
+#pragma unroll
+  for (int e = 0; e < VEC; ++e)  // one predicated store per cell: no dynamic register index
+    if (d >= 0 && d % VEC == e) rec_level[d / VEC] = has_meas ? meas_v - u.e[e] : u.e[e];
+}
+
+template <int VEC, int RY, int PD, bool REC>
 __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) {
   const int nx = A.nx, ny = A.ny;
   __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
@@ -678,21 +719,39 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
   if (y0 >= ny) return;  // whole wavefront
   const int yend = min(y0 + RY, ny);
   const int x0 = threadIdx.x * VEC;
+  const int gs = A.shot0 + shot;
   const long long so = (long long)shot * A.sstride;
   const float *__restrict__ u0 = A.u0 + so;
   const float *__restrict__ um = A.um + so;
   float *__restrict__ up1 = A.up1 + so;
   float *__restrict__ up2 = A.up2 + so;
   const float *__restrict__ pw = A.pw;
-  const int4 bb = A.bbox[A.shot0 + shot];
+  const int4 bb = A.bbox[gs];
   const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
   auto ld = [&](const float *__restrict__ base, int y) {
     const bool ok = y >= 0 && y < ny;
     return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
   };
   const int r0 = max(y0 - 1, 0), r1 = min(yend, ny - 1);
-  // the edge rows of u(k+1) (formed by the edge pass before) this strip uses
-  const Row<VEC> etop = ld(up1, r0 == 0 ? 0 : -1), ebot = ld(up1, r1 == ny - 1 ? ny - 1 : -1);
+  auto own = [&](int y) { return y >= y0 && y < yend; };
+  // receiver records: rows r0 .. r1 holding receivers, the lane's receiver
+  // descriptor read two rows ahead and its measured values one row ahead
+  unsigned long long rflags = 0;
+  if (REC) {
+#pragma unroll
+    for (int i = 0; i < RY + 2; ++i)
+      if (r0 + i <= r1 && A.row_has_recv[r0 + i]) rflags |= 1ull << i;
+  }
+  auto desc = [&](int y) {
+    const bool on = REC && y >= r0 && y <= r1 && ((rflags >> (y - r0)) & 1ull);
+    return on ? A.recv_seg[y * kBlockX + threadIdx.x] : -1;
+  };
+  const long long o1 = ((long long)gs * A.nt + A.level1) * A.nr;  // level k+1 of this shot; k+2 at + nr
+  const bool has_meas = A.meas != nullptr;
+  auto mv = [&](int d, long long o) { return (REC && d >= 0 && has_meas) ? A.meas[o + d / VEC] : 0.f; };
+  int dr = desc(r0), dn = desc(r0 + 1), dq = -1;
+  float m1r = mv(dr, o1), m2r = mv(dr, o1 + A.nr), m2q = 0.f;
+
   // queues of the next PD rows: qn[j] = u(k) row base+1+j; qm / qp / qf =
   // u(k-1), coefficients, side-edge f of row base+j
   Row<VEC> cu = ld(u0, r0 - 1), cc = ld(u0, r0);
@@ -726,27 +785,53 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
     for (int j = 0; j < PD; ++j) {
       const int r = base + j;
       if (r <= r1) {  // wavefront-uniform
+        const int d2 = desc(r + 2);
         const Row<VEC> &cn = qn[j];
         Row<VEC> v0;
+        // rows 0 and ny-1 of u(k+1), formed by the strips that use them from
+        // the inputs (loaded here: once per kernel, in the first and last
+        // strips only); the strip owning the row stores it
         if (r == 0) {
-          v0 = etop;
+          v0 = edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(A.f, 0), A.c0dt_h, x0,
+                                  nx);
+          if (own(0)) store_row<VEC>(up1 + x0, v0);
         } else if (r == ny - 1) {
-          v0 = ebot;
+          v0 = edge_row_step<VEC>(ld(u0, ny - 1), ld(u0, ny - 2), ld(u0, ny - 3), ld(um, ny - 1), ld(um, ny - 2),
+                                  ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+          if (own(ny - 1)) store_row<VEC>(up1 + (ny - 1) * nx + x0, v0);
         } else {
           v0 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, qm[j], qp[j], qf[j], r, x0, A.amp1, bb);
-          if (r >= y0 && r < yend) store_row<VEC>(up1 + r * nx + x0, v0);
+          if (own(r)) {
+            store_row<VEC>(up1 + r * nx + x0, v0);
+            if (REC && A.rec1) record_lane<VEC>(v0, dr, m1r, has_meas, A.rec + o1);
+          }
         }
         const int q = r - 1;
         if (q >= y0 && q >= 1 && q <= ny - 2) {
           const Row<VEC> w = step_row<VEC>(A, ls, s_spa, s_spab, v2, v1, v0, cu, pp, fp, q, x0, A.amp2, bb);
           store_row<VEC>(up2 + q * nx + x0, w);
+          if (REC && A.rec2) record_lane<VEC>(w, dq, m2q, has_meas, A.rec + o1 + A.nr);
         }
+        // rows 0 and ny-1 of u(k+2), by the strips owning rows 0 and ny-2
+        if (r == 2 && own(0))
+          store_row<VEC>(up2 + x0, edge_row_step<VEC>(v2, v1, v0, ld(u0, 0), cu, ld(A.f, 0), A.c0dt_h, x0, nx));
+        if (r == ny - 1 && own(ny - 2))
+          store_row<VEC>(up2 + (ny - 1) * nx + x0,
+                         edge_row_step<VEC>(v0, v1, v2, cc, cu, ld(A.f, ny - 1), A.c0dt_h, x0, nx));
+        // measured values of row r+1 (its descriptor was read a row ago)
+        const float m1n = mv(dn, o1), m2n = mv(dn, o1 + A.nr);
         v2 = v1;
         v1 = v0;
         pp = qp[j];
         fp = qf[j];
         cu = cc;
         cc = cn;
+        dq = dr;
+        dr = dn;
+        dn = d2;
+        m2q = m2r;
+        m1r = m1n;
+        m2r = m2n;
       }
     }
 #pragma unroll
@@ -1868,13 +1953,17 @@ bool forward2_supported(int nx, int sponge_w) {
 
 hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st) {
   const int vec = a.nx / kBlockX;
+  // records in the kernel: one row of read-ahead (the record pipeline keeps
+  // per-row state)
+  const bool rec = a.recv_seg != nullptr && (a.rec1 || a.rec2);
   const int ry = (rows == 8 || rows == 32) ? rows : 16;
   const int strips = (a.ny + ry - 1) / ry;
   a.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
   const dim3 grid(a.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
 #define TOMO_FWD2_PD(V, R)                                                            \
-  if (depth >= 2) forward2_kernel<V, R, 2><<<grid, block, 0, st>>>(a);                \
-  else forward2_kernel<V, R, 1><<<grid, block, 0, st>>>(a);
+  if (rec) forward2_kernel<V, R, 1, true><<<grid, block, 0, st>>>(a);                 \
+  else if (depth >= 2) forward2_kernel<V, R, 2, false><<<grid, block, 0, st>>>(a);    \
+  else forward2_kernel<V, R, 1, false><<<grid, block, 0, st>>>(a);
 #define TOMO_FWD2(V)                                                                  \
   switch (ry) {                                                                       \
     case 8: TOMO_FWD2_PD(V, 8) break;                                                 \

@@ -19,11 +19,11 @@
   and continues with plain iterations until it reaches the target or 50
   iterations in all.  Times are wall seconds of each method's loop.
 - ``train``: the training run that produced ``weights/preconditioner.npy``:
-  16 seeded 256^2 phantoms (seeds 7000-7015; 64 transducers, 2048 steps,
+  48 seeded 256^2 phantoms (seeds 7000-7047; 64 transducers, 2048 steps,
   checkpoint 200), 40 plain-SIRT iterations each at relaxation 0.1 x the
   misfit-optimal first step, every iterate a sample whose target is the
-  trajectory's best iterate (lowest error), 1500 Adam steps of batch 8 at
-  learning rate 1e-3 (seed 0).
+  trajectory's best iterate (lowest error), 4000 Adam steps of batch 8 at
+  learning rate 1e-3.
 """
 from __future__ import annotations
 
@@ -89,7 +89,7 @@ def _mask(c):
     return m.ravel()
 
 
-def train_study(out: str, phantoms: int = 16, train_steps: int = 1500, log=None) -> dict:
+def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, log=None) -> dict:
     """Reproduce the shipped network (see the module docstring; `phantoms`
     seeded phantoms from 7000); writes the parameters (float32, numpy .npy)
     to `out`."""
