@@ -143,6 +143,22 @@ bool forward2_supported(int nx, int sponge_w);
 // depth: rows whose loads are in flight ahead of the row being formed (1, 2)
 hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st);
 
+// Three forward steps k -> k+1 -> k+2 -> k+3 in one launch (same conditions
+// as the two-step kernel, and ny a multiple of the strip height): u(k+1) and
+// u(k+2) are kept in registers, u(k+3) and u(k+2) are always stored, u(k+1)
+// only when store1 is set (a recording pass needs it only at checkpoints).
+// Edge rows and, with recv_seg set, the receiver records of all three levels
+// are formed in the kernel.
+struct Fwd3Args {
+  Fwd2Args b;            // levels k-1, k in, k+1 (up1), k+2 (up2) out; records of k+1, k+2
+  float *up3;            // level k+3
+  float amp3;            // pulse * h^2 of step k+2
+  int rec3;              // record level k+3
+  int store1;            // store level k+1
+};
+bool forward3_supported(int nx, int ny, int sponge_w, int rows);
+hipError_t launch_forward3(Fwd3Args a, int nshots, int rows, hipStream_t st);
+
 // The receiver records of a pair over ring[0 .. nring) when the main kernel
 // does not write them (ring cells with a receiver index: records of levels
 // k+1 (rec1) and k+2 (rec2); ring cells without one, the top and bottom

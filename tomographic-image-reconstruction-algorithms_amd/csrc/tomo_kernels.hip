@@ -882,6 +882,168 @@ __global__ __launch_bounds__(256) void forward2_edge_kernel(Fwd2EdgeArgs A) {
   }
 }
 
+// Three leapfrog steps k -> k+1 -> k+2 -> k+3 in one pass over a strip:
+// each lane walks rows r of its strip and forms level 1 (u(k+1)) at row r,
+// level 2 at row r-1 and level 3 at row r-2, each from the three-row window
+// of the level below kept in registers (level 0 = u(k), streamed).  Level 1
+// is formed on two halo rows above and below the strip, level 2 on one.
+// Edge rows: rows 0 and ny-1 of level 1 from the inputs; of level 2 one row
+// later than its window position (the top row is patched into the window),
+// of level 3 at the last iteration that still holds its inputs.  Per three
+// cell-updates a recording pass reads u(k), u(k-1) and writes u(k+2),
+// u(k+3): 16 B instead of 24; a recompute pass also writes u(k+1).  Needs
+// ny a multiple of RY (a strip never starts one or two rows above the
+// bottom edge, whose rows it could not form).  The per-cell arithmetic is
+// that of forward_vec_kernel and forward_ring_kernel.
+template <int VEC, int RY, bool REC>
+__global__ __launch_bounds__(kBlockX *kBlockY) void forward3_kernel(Fwd3Args A3) {
+  const Fwd2Args &A = A3.b;
+  const int nx = A.nx, ny = A.ny;
+  __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
+  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
+    s_spa[i] = A.sp_a[i];
+    s_spab[i] = A.sp_ab[i];
+  }
+  __syncthreads();
+  const int lb = xcd_logical_block();
+  const int shot = lb / A.blocks_per_shot;
+  const int y0 = ((lb % A.blocks_per_shot) * kBlockY + threadIdx.y) * RY;
+  if (y0 >= ny) return;  // whole wavefront
+  const int yend = min(y0 + RY, ny);
+  const int x0 = threadIdx.x * VEC;
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
+  const float *__restrict__ u0 = A.u0 + so;
+  const float *__restrict__ um = A.um + so;
+  float *__restrict__ up1 = A.up1 + so;
+  float *__restrict__ up2 = A.up2 + so;
+  float *__restrict__ up3 = A3.up3 + so;
+  const float *__restrict__ pw = A.pw;
+  const int4 bb = A.bbox[gs];
+  const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
+  auto ld = [&](const float *__restrict__ base, int y) {
+    const bool ok = y >= 0 && y < ny;
+    return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
+  };
+  auto own = [&](int y) { return y >= y0 && y < yend; };
+  const int q1lo = max(y0 - 2, 0), q1hi = min(yend + 1, ny - 1);  // rows of level 1
+  const int q2lo = max(y0 - 1, 0), q2hi = min(yend, ny - 1);      // rows of level 2
+  const int rmax = min(yend + 1, ny);                              // last iteration
+  // receiver records: descriptors two rows ahead, measured values one ahead
+  unsigned long long rflags = 0;
+  if (REC) {
+#pragma unroll
+    for (int i = 0; i < RY + 4; ++i)
+      if (q1lo + i <= q1hi && A.row_has_recv[q1lo + i]) rflags |= 1ull << i;
+  }
+  auto desc = [&](int y) {
+    const bool on = REC && y >= q1lo && y <= q1hi && ((rflags >> (y - q1lo)) & 1ull);
+    return on ? A.recv_seg[y * kBlockX + threadIdx.x] : -1;
+  };
+  const long long o1 = ((long long)gs * A.nt + A.level1) * A.nr;  // level k+1; k+2 at + nr, k+3 at + 2 nr
+  const bool has_meas = A.meas != nullptr;
+  auto mv = [&](int d, long long o) { return (REC && d >= 0 && has_meas) ? A.meas[o + d / VEC] : 0.f; };
+  int d0 = desc(q1lo), dn = desc(q1lo + 1), d1 = -1, d2r = -1;  // rows r, r+1, r-1, r-2
+  float m1 = mv(d0, o1), m2 = 0.f, m3 = 0.f;                    // level 1 at r, 2 at r-1, 3 at r-2
+
+  // streams: u(k) rows r-1, r, r+1; u(k-1) row r; coefficients and side-edge
+  // f of rows r, r-1, r-2; windows of levels 1 (rows r-2, r-1) and 2 (rows
+  // r-3, r-2) before row r is formed
+  const Row<VEC> zero = ld(u0, -1);
+  Row<VEC> cu = ld(u0, q1lo - 1), cc = ld(u0, q1lo), cn = ld(u0, q1lo + 1), qm = ld(um, q1lo);
+  Row<VEC> p0 = ld(pw, q1lo), p1 = zero, p2 = zero;
+  float f0 = edge_f<VEC>(A.f, q1lo, x0, nx, true), f1 = 0.f, f2 = 0.f;
+  Row<VEC> w1a = zero, w1b = zero, w2a = zero, w2b = zero;
+  Row<VEC> e2bot = zero;  // row ny-1 of level 2, formed one iteration early
+#pragma unroll 1
+  for (int r = q1lo; r <= rmax; ++r) {
+    // the next row's inputs are in flight while this row computes
+    const Row<VEC> nn = ld(u0, r + 2 <= q1hi + 1 ? r + 2 : -1);
+    const Row<VEC> nm = ld(um, r + 1 <= q1hi ? r + 1 : -1);
+    const Row<VEC> np = ld(pw, r + 1 <= q1hi ? r + 1 : -1);
+    const float nf = edge_f<VEC>(A.f, r + 1, x0, nx, r + 1 <= q1hi);
+    const int d2 = desc(r + 2);
+    // level 1 at row r
+    Row<VEC> v1 = zero;
+    if (r <= q1hi) {
+      if (r == 0) {
+        v1 = edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(A.f, 0), A.c0dt_h, x0, nx);
+      } else if (r == ny - 1) {
+        v1 = edge_row_step<VEC>(ld(u0, ny - 1), ld(u0, ny - 2), ld(u0, ny - 3), ld(um, ny - 1), ld(um, ny - 2),
+                                ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+      } else {
+        v1 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, qm, p0, f0, r, x0, A.amp1, bb);
+        if (REC && A.rec1 && own(r)) record_lane<VEC>(v1, d0, m1, has_meas, A.rec + o1);
+      }
+      if (A3.store1 && own(r)) store_row<VEC>(up1 + r * nx + x0, v1);
+    }
+    // level 2 at row q = r-1
+    const int q = r - 1;
+    Row<VEC> v2 = zero;
+    if (q >= q2lo && q <= q2hi) {
+      if (q == ny - 1) {
+        v2 = e2bot;
+      } else if (q > 0) {
+        v2 = step_row<VEC>(A, ls, s_spa, s_spab, w1a, w1b, v1, cu, p1, f1, q, x0, A.amp2, bb);
+        if (own(q)) {
+          store_row<VEC>(up2 + q * nx + x0, v2);
+          if (REC && A.rec2) record_lane<VEC>(v2, d1, m2, has_meas, A.rec + o1 + A.nr);
+        }
+      }  // q == 0: formed at r = 2 (below) and patched into the window
+    }
+    // level 3 at row s = r-2
+    const int s3 = r - 2;
+    if (own(s3) && s3 > 0 && s3 < ny - 1) {
+      const Row<VEC> v3 = step_row<VEC>(A, ls, s_spa, s_spab, w2a, w2b, v2, w1a, p2, f2, s3, x0, A3.amp3, bb);
+      store_row<VEC>(up3 + s3 * nx + x0, v3);
+      if (REC && A3.rec3) record_lane<VEC>(v3, d2r, m3, has_meas, A.rec + o1 + 2 * A.nr);
+    }
+    // edge rows of levels 2 and 3
+    if (r == 2 && own(0)) {  // row 0 of level 2: level-1 rows 0, 1, 2 are w1a, w1b, v1
+      const Row<VEC> e = edge_row_step<VEC>(w1a, w1b, v1, ld(u0, 0), cu, ld(A.f, 0), A.c0dt_h, x0, nx);
+      store_row<VEC>(up2 + x0, e);
+      w2b = e;  // the window's row 0 (row r-2 after this iteration's shift)
+    }
+    if (r == 3 && own(0)) {  // row 0 of level 3: level-2 rows 0, 1, 2 are w2a, w2b, v2; level-1 row 1 is w1a
+      const Row<VEC> l1top =
+          edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(A.f, 0), A.c0dt_h, x0, nx);
+      store_row<VEC>(up3 + x0, edge_row_step<VEC>(w2a, w2b, v2, l1top, w1a, ld(A.f, 0), A.c0dt_h, x0, nx));
+    }
+    if (r == ny - 1 && q2hi == ny - 1) {  // row ny-1 of level 2: level-1 rows ny-1, ny-2, ny-3 are v1, w1b, w1a
+      e2bot = edge_row_step<VEC>(v1, w1b, w1a, cc, cu, ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+      if (own(ny - 1)) store_row<VEC>(up2 + (ny - 1) * nx + x0, e2bot);
+    }
+    if (r == ny && own(ny - 1)) {  // row ny-1 of level 3: level-2 rows ny-1, ny-2, ny-3 are v2, w2b, w2a
+      store_row<VEC>(up3 + (ny - 1) * nx + x0,
+                     edge_row_step<VEC>(v2, w2b, w2a, w1b, w1a, ld(A.f, ny - 1), A.c0dt_h, x0, nx));
+    }
+    // measured values for the next iteration: level 1 at row r+1, level 2
+    // at row r, level 3 at row r-1
+    const float m1n = mv(dn, o1), m2n = mv(d0, o1 + A.nr), m3n = mv(d1, o1 + 2 * A.nr);
+    w1a = w1b;
+    w1b = v1;
+    w2a = w2b;
+    w2b = v2;
+    cu = cc;
+    cc = cn;
+    cn = nn;
+    qm = nm;
+    p2 = p1;
+    p1 = p0;
+    p0 = np;
+    f2 = f1;
+    f1 = f0;
+    f0 = nf;
+    d2r = d1;
+    d1 = d0;
+    d0 = dn;
+    dn = d2;
+    m1 = m1n;
+    m2 = m2n;
+    m3 = m3n;
+  }
+}
+
 // Edge cells and receiver record of one forward step, for the vectorised
 // kernels (one thread per listed cell and shot; runs after forward_vec_kernel
 // on the same stream).  cells[i] = {flat cell, receiver index or -1}.
@@ -1981,6 +2143,41 @@ hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStrea
   }
 #undef TOMO_FWD2
 #undef TOMO_FWD2_PD
+  return hipGetLastError();
+}
+
+bool forward3_supported(int nx, int ny, int sponge_w, int rows) {
+  const int ry = (rows == 8 || rows == 32) ? rows : 16;
+  return forward2_supported(nx, sponge_w) && ny % ry == 0 && ny >= 2 * ry;
+}
+
+hipError_t launch_forward3(Fwd3Args a, int nshots, int rows, hipStream_t st) {
+  const int vec = a.b.nx / kBlockX;
+  const int ry = (rows == 8 || rows == 32) ? rows : 16;
+  const int strips = (a.b.ny + ry - 1) / ry;
+  a.b.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
+  const bool rec = a.b.recv_seg != nullptr && (a.b.rec1 || a.b.rec2 || a.rec3);
+  const dim3 grid(a.b.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
+#define TOMO_FWD3_R(V, R)                                                             \
+  if (rec) forward3_kernel<V, R, true><<<grid, block, 0, st>>>(a);                    \
+  else forward3_kernel<V, R, false><<<grid, block, 0, st>>>(a);
+#define TOMO_FWD3(V)                                                                  \
+  switch (ry) {                                                                       \
+    case 8: TOMO_FWD3_R(V, 8) break;                                                  \
+    case 32: TOMO_FWD3_R(V, 32) break;                                                \
+    default: TOMO_FWD3_R(V, 16) break;                                                \
+  }
+  if (vec == 8) {
+    TOMO_FWD3(8)
+  } else if (vec == 4) {
+    TOMO_FWD3(4)
+  } else if (vec == 2) {
+    TOMO_FWD3(2)
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef TOMO_FWD3
+#undef TOMO_FWD3_R
   return hipGetLastError();
 }
 
