@@ -43,11 +43,13 @@ HBM_PEAK_GBS = 8000.0
 # (roofline.achieved): the coefficient image is shared by every shot of a
 # launch (4/shots B, L2-resident); the two-step forward kernel reads u(k),
 # u(k-1) and writes u(k+1), u(k+2) once per two steps (8 B), the one-step
-# kernel 12 B; the two-step adjoint reads z(k+1), z(k+2), u(k), u(k-1) and
-# the per-shot gradient and writes z(k), z(k-1) and the gradient once per two
-# steps (16 B), the one-step adjoint re-reads and re-writes the gradient each
-# step (24 B).
-FWD_BYTES = {1: 12.0, 2: 8.0}
+# kernel 12 B; with three steps a recording pass writes only u(k+2), u(k+3)
+# (16 B per three updates) and a recompute pass all three levels (20 B per
+# three), 6 B on average over the two; the two-step adjoint reads z(k+1),
+# z(k+2), u(k), u(k-1) and the per-shot gradient and writes z(k), z(k-1) and
+# the gradient once per two steps (16 B), the one-step adjoint re-reads and
+# re-writes the gradient each step (24 B).
+FWD_BYTES = {1: 12.0, 2: 8.0, 3: 6.0}
 ADJ_BYTES = {1: 24.0, 2: 16.0}
 # The per-unit figures of the one-step formulation, reported beside them: a
 # forward cell-update reads u(k), u(k-1) and the per-cell coefficient and
@@ -166,8 +168,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of --study train")
     ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
-    ap.add_argument("--time-blocking", type=int, default=2, choices=[1, 2],
-                    help="forward steps per launch (2: two steps with the intermediate level in registers)")
+    ap.add_argument("--time-blocking", type=int, default=3, choices=[1, 2, 3],
+                    help="forward steps per launch (2, 3: the intermediate levels in registers)")
     ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows,fwd_depth,adj_depth (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--data", default="fp64", choices=["fp64", "fp32"],
@@ -265,7 +267,7 @@ def main():
     adj_gbs = ADJ_BYTES[ab] * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
     fwd_unit_gbs = FWD_UNIT_BYTES * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
     adj_unit_gbs = ADJ_UNIT_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
-    fwd_kernel = "forward2_kernel" if tb == 2 else "forward_vec_kernel"
+    fwd_kernel = {1: "forward_vec_kernel", 2: "forward2_kernel", 3: "forward3_kernel"}[tb]
     adj_kernel = "adjoint2_kernel" if ab == 2 else "adjoint_vec_kernel"
     fwd_traffic = load_traffic(fwd_kernel, shape, tb) if args.workload == "c3" else None
     adj_traffic = load_traffic(adj_kernel, shape, ab) if args.workload == "c3" else None

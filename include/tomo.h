@@ -326,12 +326,16 @@ int tomo_timing_reset(tomo_ctx *ctx);
  * sirt-mimo.cu:165-299). */
 int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_depth, int adj_depth);
 
-/* Stencil steps per launch in the gradient passes: 2 (default) runs two
- * leapfrog steps in one pass over each strip (the intermediate level kept in
+/* Stencil steps per launch in the forward passes: 2 runs two leapfrog
+ * steps in one pass over each strip (the intermediate level kept in
  * registers: 8 instead of 12 bytes of traffic per forward cell-update) where
  * it applies -- the sponge preset, vectorised kernels, nx of 128, 256 or
- * 512, checkpoint interval >= 4 -- and 1 otherwise.  Where the forward pass
- * runs two steps, the adjoint sweep does too (16 instead of 24 bytes per
+ * 512, checkpoint interval >= 4 -- and 1 otherwise.  3 (default) runs three
+ * forward steps per launch where, in addition, the strips tile the rows
+ * exactly and the checkpoint interval is >= 5 (a recording pass then writes
+ * only the last two levels of each launch: 16 bytes per three
+ * cell-updates), else two or one.  Where the forward pass runs two or three
+ * steps, the adjoint sweep runs two (16 instead of 24 bytes per
  * cell-update) when the receivers are points with additive injection, no
  * 64th of a row holds two of them, and coverage is off.  Results are the
  * same bit for bit.  tomo_get_time_blocking returns the forward steps per
@@ -349,7 +353,8 @@ int tomo_get_adjoint_blocking(const tomo_ctx *ctx);
  * the lowest gradient-pass cost (forward time x forward launches per
  * adjoint launch + adjoint time).  Where two steps per launch apply
  * (tomo_set_time_blocking), the strip heights are then re-timed on the
- * two-step kernels, which run all but the odd tails.  The result is set as
+ * two-step kernels (the forward's on the three-step kernel where that
+ * applies), which run all but the tails.  The result is set as
  * the context's shape and cached per size class for the process (force = 1
  * re-times).  out5 (may be NULL) = {vec, fwd_rows, adj_rows, fwd_depth,
  * adj_depth}; ms2 (may be NULL) = {forward, adjoint} ms per launch of one

@@ -1,5 +1,5 @@
-"""Two stencil steps per launch (the intermediate level held in registers),
-forward and adjoint, against one step per launch: traces, misfits and
+"""Two and three stencil steps per launch (the intermediate levels held in
+registers; two for the adjoint), against one step per launch: traces, misfits and
 gradients must be bitwise equal -- the per-cell arithmetic is the same -- for
 each full-row lane width (nx = 128, 256, 512 -> 2, 4, 8 cells per lane), with
 an odd number of steps (a single-step tail), checkpoint segments of odd and
@@ -49,28 +49,32 @@ def compare(e1, e2, c, shots, coverage=False):
         np.testing.assert_array_equal(e2.get_coverage(), e1.get_coverage())
 
 
+@pytest.mark.parametrize("steps", [2, 3])
 @pytest.mark.parametrize("n,nt,ckpt", [(128, 301, 0), (256, 301, 49), (512, 203, 50), (512, 160, 7)])
-def test_two_step_launches_match_single_steps(n, nt, ckpt):
-    print(f"[case] time blocking n={n} nt={nt} ckpt={ckpt}", flush=True)
+def test_blocked_launches_match_single_steps(n, nt, ckpt, steps):
+    print(f"[case] time blocking {steps} n={n} nt={nt} ckpt={ckpt}", flush=True)
     e1, c = engine(n, nt, ckpt, 1)
-    e2, _ = engine(n, nt, ckpt, 2)
+    e2, _ = engine(n, nt, ckpt, steps)
     try:
-        assert e1.time_blocking() == 1 and e2.time_blocking() == 2
+        assert e1.time_blocking() == 1 and e2.time_blocking() == steps
         assert e1.adjoint_blocking() == 1 and e2.adjoint_blocking() == 2
         compare(e1, e2, c, 6)
+        # the FP32 trace forward (its own rolling ring) agrees too
+        np.testing.assert_array_equal(e2.forward(fp64=False), e1.forward(fp64=False))
     finally:
         e1.close()
         e2.close()
 
 
-def test_many_receivers_on_strip_and_halo_rows():
+@pytest.mark.parametrize("steps", [2, 3])
+def test_many_receivers_on_strip_and_halo_rows(steps):
     """128 sources and receivers on the ring at 256^2 (receivers on most
-    rows, so on strip boundaries and halo rows of both levels)."""
-    print("[case] time blocking 256 with 128 receivers", flush=True)
+    rows, so on strip boundaries and halo rows of every level)."""
+    print(f"[case] time blocking {steps}, 256 with 128 receivers", flush=True)
     e1, c = engine(256, 241, 60, 1, shots=128)
-    e2, _ = engine(256, 241, 60, 2, shots=128)
+    e2, _ = engine(256, 241, 60, steps, shots=128)
     try:
-        assert e2.adjoint_blocking() == 2
+        assert e2.time_blocking() == steps and e2.adjoint_blocking() == 2
         compare(e1, e2, c, 128)
     finally:
         e1.close()
@@ -88,14 +92,16 @@ def test_fallback_to_one_step_adjoint():
     finally:
         e1.close()
         e2.close()
-    # two receivers in one 4-cell lane segment of a row
+    # two receivers in one 4-cell lane segment of a row: records in passes
+    # after the blocked forward launches, one-step adjoint
     ri = np.array([100, 101, 150], np.int32)
     rj = np.array([120, 120, 60], np.int32)
-    e1, c = engine(256, 201, 40, 1, receivers=(ri, rj))
-    e2, _ = engine(256, 201, 40, 2, receivers=(ri, rj))
-    try:
-        assert e2.time_blocking() == 2 and e2.adjoint_blocking() == 1
-        compare(e1, e2, c, 6)
-    finally:
-        e1.close()
-        e2.close()
+    for steps in (2, 3):
+        e1, c = engine(256, 201, 40, 1, receivers=(ri, rj))
+        e2, _ = engine(256, 201, 40, steps, receivers=(ri, rj))
+        try:
+            assert e2.time_blocking() == steps and e2.adjoint_blocking() == 1
+            compare(e1, e2, c, 6)
+        finally:
+            e1.close()
+            e2.close()

@@ -449,7 +449,7 @@ class Engine:
         return {"ms": out[0], "launches": int(out[1]), "cells": out[2]}
 
     def set_time_blocking(self, steps: int):
-        """Forward steps per launch in the gradient passes (1 or 2)."""
+        """Forward steps per launch in the forward passes (1, 2 or 3; default 3)."""
         self._c(lib().tomo_set_time_blocking(self._ctx, int(steps)))
 
     def time_blocking(self) -> int:

@@ -75,7 +75,7 @@ struct tomo_ctx {
   std::vector<TimedSpan> spans;
   double t_ms[3] = {0, 0, 0}, t_launch[3] = {0, 0, 0}, t_cells[3] = {0, 0, 0};
   // forward steps per launch of the gradient passes (1, or 2 where supported)
-  int time_block = 2;
+  int time_block = 3;
   // per-pixel illumination weights of the updates (0 off, 1 coverage)
   int pixw_mode = 0;
   double pixw_eps = 0.05;

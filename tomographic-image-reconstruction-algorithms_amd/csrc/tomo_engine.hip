@@ -283,7 +283,7 @@ int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
   const size_t lvl = (size_t)P * c->n2;
   size_t seg_levels, ckpt_levels;
   if (!need_history) {
-    seg_levels = 4;  // rolling levels of a forward pass (4 with two steps per launch)
+    seg_levels = 5;  // rolling levels of a forward pass (ring_levels: at most 5)
     ckpt_levels = 0;
   } else if (full_history(c)) {
     seg_levels = (size_t)c->nt;
@@ -319,6 +319,19 @@ bool two_step(const tomo_ctx *c) {
   return full_history(c) || c->cfg.checkpoint_every >= 4;
 }
 
+// three forward steps per launch: the two-step conditions, strips that tile
+// the rows exactly, and checkpoint segments of at least 5 levels
+bool three_step(const tomo_ctx *c) {
+  if (c->time_block < 3 || !two_step(c)) return false;
+  if (!forward3_supported(c->nx, c->ny, c->cfg.sponge_width, stencil_shape(c).fwd_rows)) return false;
+  return full_history(c) || c->cfg.checkpoint_every >= 5;
+}
+
+// rolling levels of a forward pass: a launch of m steps writes levels k+1 ..
+// k+m while other strips still read levels k-1 and k on their halo rows, so
+// the ring holds m + 2 levels
+int ring_levels(const tomo_ctx *c) { return three_step(c) ? 5 : two_step(c) ? 4 : 3; }
+
 // two adjoint steps per launch: with two forward steps per launch, point
 // receivers with additive injection, at most one receiver per lane segment,
 // and no coverage accumulation
@@ -328,8 +341,7 @@ bool two_step_adjoint(const tomo_ctx *c) {
 }
 
 // location of forward level k during the first (recording) pass (the
-// rolling levels use a ring of 4 when two steps share a launch: level k+2 is
-// written while other strips still read level k-1)
+// rolling levels use a ring of ring_levels(c))
 float *pass1_level(tomo_ctx *c, int P, int k, bool need_history) {
   const size_t lvl = (size_t)P * c->n2;
   if (!need_history) return c->d_seg + (size_t)(k % 3) * lvl;
@@ -337,7 +349,7 @@ float *pass1_level(tomo_ctx *c, int P, int k, bool need_history) {
   const int C = c->cfg.checkpoint_every;
   const int q = k % C;
   if (q < 2) return c->d_ckpt + ((size_t)(k / C) * 2 + q) * lvl;
-  return c->d_seg + (size_t)(k % (two_step(c) ? 4 : 3)) * lvl;
+  return c->d_seg + (size_t)(k % ring_levels(c)) * lvl;
 }
 
 // location of level k while segment m is being replayed
@@ -464,6 +476,81 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
   return TOMO_OK;
 }
 
+// Forward steps k, k+1, k+2 (levels k-1, k -> k+1, k+2, k+3) in one launch,
+// edge rows included; level k+1 is stored when store1 is set.  Receiver
+// records in the same launch when every receiver has its own lane segment,
+// else (level k+1 then stored) in small passes after.
+int forward_step3(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u0, float *up1, float *up2,
+                  float *up3, int k, int P, bool record, const float *meas, float *rec, bool store1) {
+  const bool rec_on = record && rec != nullptr && c->nr > 0;
+  Fwd3Args t{};
+  Fwd2Args &b = t.b;
+  b.um = um;
+  b.u0 = u0;
+  b.up1 = up1;
+  b.up2 = up2;
+  b.sstride = a.sstride;
+  b.pw = a.pw;
+  b.f = a.f;
+  b.bbox = a.bbox;
+  b.nx = c->nx;
+  b.ny = c->ny;
+  b.shot0 = a.shot0;
+  b.amp1 = pulse_h2<float>(c, k);
+  b.amp2 = pulse_h2<float>(c, k + 1);
+  t.amp3 = pulse_h2<float>(c, k + 2);
+  b.c0dt_h = a.c0dt_h;
+  b.sponge_w = a.sponge_w;
+  b.sp_a = a.sp_a;
+  b.sp_ab = a.sp_ab;
+  t.up3 = up3;
+  b.rec1 = rec_on && k + 1 >= c->cfg.record_first;
+  b.rec2 = rec_on && k + 2 >= c->cfg.record_first;
+  t.rec3 = rec_on && k + 3 >= c->cfg.record_first;
+  const bool any = b.rec1 || b.rec2 || t.rec3;
+  const bool in_kernel = any && c->recvseg_ok;
+  t.store1 = store1 || (any && !in_kernel);
+  if (in_kernel) {
+    b.recv_seg = c->d_recvseg;
+    b.row_has_recv = c->d_rowflag;
+    b.meas = meas;
+    b.rec = rec;
+    b.nt = c->nt;
+    b.nr = c->nr;
+    b.level1 = k + 1;
+  }
+  HIPCHK(c, launch_forward3(t, P, stencil_shape(c).fwd_rows, c->stream));
+  if (any && !in_kernel) {  // records over the receivers: levels k+1, k+2, then k+3
+    Fwd2EdgeArgs e{};
+    e.sstride = a.sstride;
+    e.f = a.f;
+    e.ring = c->d_ring + 2 * c->nx;
+    e.nring = c->nr;
+    e.nx = c->nx;
+    e.ny = c->ny;
+    e.nt = c->nt;
+    e.nr = c->nr;
+    e.shot0 = a.shot0;
+    e.meas = meas;
+    e.rec = rec;
+    e.c0dt_h = a.c0dt_h;
+    e.u0 = u0;
+    e.up1 = up1;
+    e.up2 = up2;
+    e.level1 = k + 1;
+    e.rec1 = b.rec1;
+    e.rec2 = b.rec2;
+    HIPCHK(c, launch_forward2_edges(e, P, c->stream));
+    e.up1 = up3;
+    e.up2 = up3;
+    e.level1 = k + 3;
+    e.rec1 = t.rec3;
+    e.rec2 = 0;
+    HIPCHK(c, launch_forward2_edges(e, P, c->stream));
+  }
+  return TOMO_OK;
+}
+
 // traces of shots [g0, g0+ng) into d_rr (as u values, or the residuals
 // meas - u when meas is given), float or double solver
 template <typename T>
@@ -476,9 +563,13 @@ int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const 
     HIPCHK(c, launch_fill_f32(c->d_rr + (size_t)p0 * tr, (size_t)np * tr, 0.f, c->stream));
     // FP32 with two steps per launch: a ring of 4 levels (level k+2 is written
     // while other strips still read level k-1 on their halo rows)
-    bool two = false;
-    if constexpr (std::is_same<T, float>::value) two = two_step(c);
-    auto L = [&](int k) { return ring + (size_t)(k % (two ? 4 : 3)) * ring_lvl; };
+    bool two = false, three = false;
+    if constexpr (std::is_same<T, float>::value) {
+      two = two_step(c);
+      three = three_step(c);
+    }
+    const int R = three ? 5 : two ? 4 : 3;
+    auto L = [&](int k) { return ring + (size_t)(k % R) * ring_lvl; };
     if (std::is_same<T, float>::value) {
       HIPCHK(c, launch_fill_f32((float *)L(0), (size_t)np * c->n2, 0.f, c->stream));
       HIPCHK(c, launch_fill_f32((float *)L(1), (size_t)np * c->n2, 0.f, c->stream));
@@ -492,6 +583,13 @@ int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const 
     for (int k = 1; k < c->nt - 1;) {
       int rc;
       if constexpr (std::is_same<T, float>::value) {
+        if (three && k + 2 < c->nt - 1) {
+          rc = forward_step3(c, a, L(k - 1), L(k), L(k + 1), L(k + 2), L(k + 3), k, np, true, meas, c->d_rr,
+                             false);
+          if (rc) return rc;
+          k += 3;
+          continue;
+        }
         if (two && k + 1 < c->nt - 1) {
           rc = forward_step2(c, a, L(k - 1), L(k), L(k + 1), L(k + 2), k, np, true, meas, c->d_rr);
           if (rc) return rc;
@@ -567,10 +665,18 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   {
     TimedSpan s;
     span_begin(c, s, 0);
-    const bool two = two_step(c);
+    const bool two = two_step(c), three = three_step(c);
+    const int C = full_history(c) ? 0 : c->cfg.checkpoint_every;
     for (int k = 1; k < nt - 1;) {
       int rc;
-      if (two && k + 1 < nt - 1) {
+      if (three && k + 2 < nt - 1) {
+        // level k+1 stays in registers unless the history keeps it
+        const bool store1 = C == 0 || (k + 1) % C < 2;
+        rc = forward_step3(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
+                           pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true),
+                           pass1_level(c, P, k + 3, true), k, np, true, c->d_meas, c->d_rr, store1);
+        k += 3;
+      } else if (two && k + 1 < nt - 1) {
         rc = forward_step2(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
                            pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true), k, np, true, c->d_meas,
                            c->d_rr);
@@ -607,11 +713,16 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
     if (!full && cnt > 2) {
       TimedSpan s;
       span_begin(c, s, 0);
-      const bool two = two_step(c);
+      const bool two = two_step(c), three = three_step(c);
       for (int q = 2; q < cnt;) {
         const int k = k0 + q - 1;
         int rc;
-        if (two && q + 1 < cnt) {
+        if (three && q + 2 < cnt) {
+          rc = forward_step3(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
+                             segment_level(c, P, m, k + 1), segment_level(c, P, m, k + 2),
+                             segment_level(c, P, m, k + 3), k, np, false, nullptr, nullptr, true);
+          q += 3;
+        } else if (two && q + 1 < cnt) {
           rc = forward_step2(c, fa, segment_level(c, P, m, k - 1), segment_level(c, P, m, k),
                              segment_level(c, P, m, k + 1), segment_level(c, P, m, k + 2), k, np, false, nullptr,
                              nullptr);
@@ -676,7 +787,7 @@ std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
                          (uintptr_t)c->d_rowflag, (uintptr_t)c->d_spa, (uintptr_t)c->d_spab,
                          (uintptr_t)c->coverage, (uintptr_t)c->nshots, (uintptr_t)c->nr, (uintptr_t)sh.vec,
                          (uintptr_t)sh.fwd_rows, (uintptr_t)sh.adj_rows, (uintptr_t)sh.fwd_depth,
-                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c),
+                         (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c), (uintptr_t)three_step(c),
                          (uintptr_t)two_step_adjoint(c), (uintptr_t)c->d_recvseg};
   return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
 }
@@ -1344,12 +1455,14 @@ int tomo_line_search(tomo_ctx *c, int weighted, double s_lo, double s_step, int 
 
 int tomo_set_time_blocking(tomo_ctx *c, int steps) {
   if (!c) return TOMO_ERR_ARG;
-  if (steps != 1 && steps != 2) return fail(c, TOMO_ERR_ARG, "steps per launch must be 1 or 2");
+  if (steps < 1 || steps > 3) return fail(c, TOMO_ERR_ARG, "steps per launch must be 1, 2 or 3");
   c->time_block = steps;
   return TOMO_OK;
 }
 
-int tomo_get_time_blocking(const tomo_ctx *c) { return c ? (two_step(c) ? 2 : 1) : TOMO_ERR_ARG; }
+int tomo_get_time_blocking(const tomo_ctx *c) {
+  return c ? (three_step(c) ? 3 : two_step(c) ? 2 : 1) : TOMO_ERR_ARG;
+}
 int tomo_get_adjoint_blocking(const tomo_ctx *c) { return c ? (two_step_adjoint(c) ? 2 : 1) : TOMO_ERR_ARG; }
 
 int tomo_set_pixel_weights(tomo_ctx *c, int mode, double eps) {
@@ -1536,6 +1649,18 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
           }, &t);
           if (rc) break;
           if (t < tf) { tf = t; df = depth; }
+        }
+        // where three steps per launch apply at this strip height, those
+        // kernels run all but the tails: they are timed instead (a recording
+        // launch, level k+1 dropped, and a replay launch storing all three)
+        if (!rc && three_step(c)) {
+          float *U3 = pass1_level(c, P, 2, true);
+          double t = 0.0;
+          rc = time_launches(c, reps, [&]() -> int {
+            int r = forward_step3(c, fa, Z0, Z1, Z2, Z3, U3, mid, P, false, nullptr, nullptr, false);
+            return r ? r : forward_step3(c, fa, Z0, Z1, Z2, Z3, U3, mid, P, false, nullptr, nullptr, true);
+          }, &t);
+          tf = t / 3;  // two launches of three levels, per two levels as below
         }
         if (!rc && adj2) {
           Adj2Args b = adj2_args(c, 0);
