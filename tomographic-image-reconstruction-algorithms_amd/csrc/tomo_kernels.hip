@@ -381,22 +381,24 @@ struct Row {
 template <int VEC>
 struct LaneSponge {
   float a[VEC], ab[VEC];
-  int dx[VEC];
+  int x0, nx;  // the distance of cell e to the side edges is recomputed where needed
 };
 
 template <int VEC>
 __device__ __forceinline__ LaneSponge<VEC> lane_sponge(int x0, int nx, int w, const float *__restrict__ sp_a,
                                                        const float *__restrict__ sp_ab, bool ok) {
   LaneSponge<VEC> s;
+  s.x0 = x0;
+  s.nx = nx;
 #pragma unroll
   for (int e = 0; e < VEC; ++e) {
     const int x = x0 + e;
-    s.dx[e] = min(x, nx - 1 - x);
+    const int dx = min(x, nx - 1 - x);
     s.a[e] = 1.f;
     s.ab[e] = 1.f;
-    if (ok && s.dx[e] < w) {
-      s.a[e] = sp_a[s.dx[e]];
-      s.ab[e] = sp_ab[s.dx[e]];
+    if (ok && dx < w) {
+      s.a[e] = sp_a[dx];
+      s.ab[e] = sp_ab[dx];
     }
   }
   return s;
@@ -410,7 +412,8 @@ __device__ __forceinline__ void sponge_at(const LaneSponge<VEC> &s, int e, int y
     a = s.a[e];
     ab = s.ab[e];
   } else {
-    const int d = min(s.dx[e], dy);
+    const int x = s.x0 + e;
+    const int d = min(min(x, s.nx - 1 - x), dy);
     a = sp_a[d];
     ab = sp_ab[d];
   }
@@ -896,10 +899,14 @@ __global__ __launch_bounds__(256) void forward2_edge_kernel(Fwd2EdgeArgs A) {
 // bottom edge, whose rows it could not form).  The per-cell arithmetic is
 // that of forward_vec_kernel and forward_ring_kernel.
 template <int VEC, int RY, bool REC>
-__global__ __launch_bounds__(kBlockX *kBlockY) void forward3_kernel(Fwd3Args A3) {
+__global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_eu(2))) void forward3_kernel(
+    Fwd3Args A3) {
   const Fwd2Args &A = A3.b;
   const int nx = A.nx, ny = A.ny;
   __shared__ float s_spa[kSpongeLds], s_spab[kSpongeLds];
+  // row ny-1 of level 2, formed one iteration before it is used: a lane-private
+  // LDS slot instead of VEC registers held through the loop
+  __shared__ float s_e2[kBlockY][VEC][kBlockX];
   for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY) {
     s_spa[i] = A.sp_a[i];
     s_spab[i] = A.sp_ab[i];
@@ -954,7 +961,6 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward3_kernel(Fwd3Args A3)
   Row<VEC> p0 = ld(pw, q1lo), p1 = zero, p2 = zero;
   float f0 = edge_f<VEC>(A.f, q1lo, x0, nx, true), f1 = 0.f, f2 = 0.f;
   Row<VEC> w1a = zero, w1b = zero, w2a = zero, w2b = zero;
-  Row<VEC> e2bot = zero;  // row ny-1 of level 2, formed one iteration early
 #pragma unroll 1
   for (int r = q1lo; r <= rmax; ++r) {
     // the next row's inputs are in flight while this row computes
@@ -982,7 +988,8 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward3_kernel(Fwd3Args A3)
     Row<VEC> v2 = zero;
     if (q >= q2lo && q <= q2hi) {
       if (q == ny - 1) {
-        v2 = e2bot;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v2.e[e] = s_e2[threadIdx.y][e][threadIdx.x];
       } else if (q > 0) {
         v2 = step_row<VEC>(A, ls, s_spa, s_spab, w1a, w1b, v1, cu, p1, f1, q, x0, A.amp2, bb);
         if (own(q)) {
@@ -1010,8 +1017,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward3_kernel(Fwd3Args A3)
       store_row<VEC>(up3 + x0, edge_row_step<VEC>(w2a, w2b, v2, l1top, w1a, ld(A.f, 0), A.c0dt_h, x0, nx));
     }
     if (r == ny - 1 && q2hi == ny - 1) {  // row ny-1 of level 2: level-1 rows ny-1, ny-2, ny-3 are v1, w1b, w1a
-      e2bot = edge_row_step<VEC>(v1, w1b, w1a, cc, cu, ld(A.f, ny - 1), A.c0dt_h, x0, nx);
-      if (own(ny - 1)) store_row<VEC>(up2 + (ny - 1) * nx + x0, e2bot);
+      const Row<VEC> e = edge_row_step<VEC>(v1, w1b, w1a, cc, cu, ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) s_e2[threadIdx.y][i][threadIdx.x] = e.e[i];
+      if (own(ny - 1)) store_row<VEC>(up2 + (ny - 1) * nx + x0, e);
     }
     if (r == ny && own(ny - 1)) {  // row ny-1 of level 3: level-2 rows ny-1, ny-2, ny-3 are v2, w2b, w2a
       store_row<VEC>(up3 + (ny - 1) * nx + x0,
