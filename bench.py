@@ -297,6 +297,7 @@ def main():
         "forward": {
             "kernel": f"{fwd_kernel} (with its edge/receiver passes)",
             "achieved": round(fwd_gbs, 1), "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": fwd_traffic,
+            "traffic_of": "recording launches" if tb == 3 else "every launch",
             "bytes_per_cell_update": FWD_BYTES[tb], "cells_per_launch": fwd_cells_launch,
             "avg_launch_ms": round(fwd_avg_ms, 5), "total_ms": round(tm["fwd_ms"], 2),
             "unit_bytes_per_cell_update": FWD_UNIT_BYTES, "unit_achieved": round(fwd_unit_gbs, 1),
