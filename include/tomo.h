@@ -272,6 +272,15 @@ int tomo_update_uniform(tomo_ctx *ctx, float scale, int mode);
  * weights_out (may be NULL) receives the normalised weights, double[Ng]. */
 int tomo_update_weighted(tomo_ctx *ctx, float scale, double *weights_out);
 
+/* Weighted average of intermediate fields (sirt-mimo-avg-weight.cu:80-203,
+ * 530-544): fg_g = f + scale * mask * G_g / (1 + f) per group;
+ * w_g = || fg_g - mean_g fg_g ||_2 over the full grid; f += sum_g fg_g / w_g
+ * (the whole intermediate fields, f included, as the reference adds them).
+ * All sums FP64.  A zero w_g (one group, or identical groups) leaves a
+ * non-finite field, as the reference's division does.  weights_out (may be
+ * NULL) receives double[Ng + 1]: w_1 .. w_Ng and their sum. */
+int tomo_update_avg_weight(tomo_ctx *ctx, float scale, double *weights_out);
+
 /* ART step for one shot (art-mimo-io.cu:381-400, 684):
  * f += scale * mask * G_g / (1 + f). */
 int tomo_update_group(tomo_ctx *ctx, int g, float scale);

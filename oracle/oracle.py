@@ -28,6 +28,10 @@ checkpointing              bitwise equality with the single-segment path
 orc_update_uniform_*       closed form on constant gradients; KAT scale s*
 orc_update_weighted_*      weights sum to 1, permutation symmetry, degenerate
                            fallback to 1/M; weighted data_dynamic KATs
+orc_update_avg_weight_*    closed form for two groups (w_1 = w_2 = s ||mask (df_1 -
+                           df_2)|| / 2, f += (fg_1 + fg_2) / w); permutation symmetry;
+                           f enters the update whole (the reference's rule); one
+                           group gives a non-finite field, as in the reference
 orc_update_group_*         closed form
 orc_error_*                closed form
 orc_schedule               closed forms of both schedules
@@ -93,6 +97,8 @@ def lib():
             fn = getattr(L, "orc_update_uniform_" + suf)
             fn.argtypes = [vp, p, ctypes.c_int, p, ctypes.c_double, ctypes.c_int]
             fn = getattr(L, "orc_update_weighted_" + suf)
+            fn.argtypes = [vp, p, ctypes.c_int, p, ctypes.c_double, vp]
+            fn = getattr(L, "orc_update_avg_weight_" + suf)
             fn.argtypes = [vp, p, ctypes.c_int, p, ctypes.c_double, vp]
             fn = getattr(L, "orc_update_group_" + suf)
             fn.argtypes = [vp, p, p, ctypes.c_double]
@@ -222,6 +228,16 @@ def update_weighted(cfg, f, G, scale):
     w = np.zeros(G.shape[0], np.float64)
     getattr(lib(), "orc_update_weighted_" + suf)(ctypes.byref(cfg), f, G.shape[0], np.ascontiguousarray(G, t),
                                                   float(scale), _ptr(w))
+    return f, w
+
+
+def update_avg_weight(cfg, f, G, scale):
+    """Weighted average of the intermediate fields (sirt-mimo-avg-weight.cu:
+    80-203): returns f and [w_1 .. w_M, sum w]."""
+    t, suf = _dt(f.dtype)
+    w = np.zeros(G.shape[0] + 1, np.float64)
+    getattr(lib(), "orc_update_avg_weight_" + suf)(ctypes.byref(cfg), f, G.shape[0],
+                                                    np.ascontiguousarray(G, t), float(scale), _ptr(w))
     return f, w
 
 

@@ -545,6 +545,49 @@ void update_weighted(const tomo_config *c, T *f, int M, const T *G, double scale
     }
 }
 
+// Weighted average of intermediate fields (sirt-mimo-avg-weight.cu:80-203,
+// 530-544), step by step as the reference runs it:
+//   fg_g = f + scale mask G_g / (1 + f)           (update_intermediate_fields)
+//   favg = sum_g fg_g / M                           (sum_ / average_intermediate_fields)
+//   w_g  = || fg_g - favg ||_2 over the full grid   (get_normalized_weights)
+//   f   += sum_g fg_g / w_g                         (update_weighted_average_field, update_field)
+// The whole intermediate fields, f included, are added.  A zero w_g (one
+// group, or identical groups) makes f non-finite, as in the reference.
+// wout (may be NULL) receives w_g and, in wout[M], their sum (get_total_weight).
+template <typename T>
+void update_avg_weight(const tomo_config *c, T *f, int M, const T *G, double scale, double *wout) {
+  const int nx = c->nx, ny = c->ny, n2 = nx * ny;
+  auto fg = [&](int m, int id) {
+    const int i = id % nx, j = id / nx;
+    const double df = double(G[(size_t)m * n2 + id]) / (1.0 + double(f[id]));
+    return double(f[id]) + (in_mask(*c, i, j) ? scale * df : 0.0);
+  };
+  std::vector<double> favg(n2, 0.0), w(M, 0.0);
+  for (int id = 0; id < n2; ++id) {
+    double s = 0.0;
+    for (int m = 0; m < M; ++m) s += fg(m, id);
+    favg[id] = s / M;
+  }
+  double total = 0.0;
+  for (int m = 0; m < M; ++m) {
+    double acc = 0.0;
+    for (int id = 0; id < n2; ++id) {
+      const double v = fg(m, id) - favg[id];
+      acc += v * v;
+    }
+    w[m] = std::sqrt(acc);
+    total += w[m];
+  }
+  if (wout) {
+    for (int m = 0; m < M; ++m) wout[m] = w[m];
+    wout[M] = total;
+  }
+  std::vector<double> add(n2, 0.0);
+  for (int id = 0; id < n2; ++id)
+    for (int m = 0; m < M; ++m) add[id] += fg(m, id) / w[m];
+  for (int id = 0; id < n2; ++id) f[id] = T(double(f[id]) + add[id]);
+}
+
 // ART step (art-mimo-io.cu:381-400)
 template <typename T>
 void update_group(const tomo_config *c, T *f, const T *Gg, double scale) {
@@ -586,6 +629,10 @@ double error_pct(const T *f, const float *fo, int n2) {
   extern "C" void orc_update_weighted_##SUF(const tomo_config *c, T *f, int nshots, const T *G,    \
                                             double scale, double *w) {                             \
     update_weighted<T>(c, f, nshots, G, scale, w);                                                 \
+  }                                                                                                \
+  extern "C" void orc_update_avg_weight_##SUF(const tomo_config *c, T *f, int nshots, const T *G,  \
+                                              double scale, double *w) {                           \
+    update_avg_weight<T>(c, f, nshots, G, scale, w);                                               \
   }                                                                                                \
   extern "C" void orc_update_group_##SUF(const tomo_config *c, T *f, const T *Gg, double scale) {  \
     update_group<T>(c, f, Gg, scale);                                                              \

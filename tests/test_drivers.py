@@ -2,7 +2,8 @@
 file names, text layout and stop rules (generate-field.cu, generate-data.cu,
 sirt-mimo-io.cu, sirt-mimo-norm-mem.cu, art-mimo-io.cu, sirt-mimo-seq.cu,
 sirt-test-weight.cu, and the averaging / group-serial variants
-sirt-mimo-avg.cu, sirt-mimo-weight.cu, sirt-iter.cu, sirt-iter-norm.cu)."""
+sirt-mimo-avg.cu, sirt-mimo-avg-weight.cu, sirt-mimo-weight.cu, sirt-iter.cu,
+sirt-iter-norm.cu)."""
 import os
 import subprocess
 
@@ -35,9 +36,10 @@ def test_usage_and_filename_errors(tmp_path):
     assert r.returncode == 1 and "Usage" in r.stderr
     r = run(["sirt-mimo-io", "a.b.txt", 10, 1, 1, 1, 1, 1, 1], tmp_path)
     assert r.returncode == 1 and "only one period" in r.stderr
-    # the variants' argument lists (sirt-mimo-avg.cu:113-115,
+    # the variants' argument lists (sirt-mimo-avg.cu:113-115, sirt-mimo-avg-weight.cu:210-212,
     # sirt-mimo-weight.cu:68-70, sirt-iter.cu:410-412, sirt-iter-norm.cu:464-466)
-    for exe, extra in (("sirt-mimo-avg", ""), ("sirt-mimo-weight", " <alpha> <beta> <omega>"),
+    for exe, extra in (("sirt-mimo-avg", ""), ("sirt-mimo-avg-weight", ""),
+                       ("sirt-mimo-weight", " <alpha> <beta> <omega>"),
                        ("sirt-iter", " <omega> <alpha>"), ("sirt-iter-norm", " <omega> <alpha>")):
         r = run([exe, "fo.txt", 10], tmp_path)
         assert r.returncode == 1
@@ -78,6 +80,19 @@ def test_averaging_and_group_serial_variants(tmp_path):
         got = [float(l.split("=")[1]) for l in r.stdout.splitlines() if l.startswith("epsilon =")]
         np.testing.assert_allclose(got, engine_eps([20000.0, 20000.0], False), rtol=1e-6)
         assert os.path.exists(tmp_path / "sirt-160-fo.txt")
+        # distance-weighted sum of the intermediate fields at 20000, as far as
+        # the driver's stop rule lets it run
+        r = run(["sirt-mimo-avg-weight", "fo.txt", 160, 0, 2], tmp_path)
+        assert r.returncode == 0, r.stderr
+        got = [float(l.split("=")[1]) for l in r.stdout.splitlines() if l.startswith("epsilon =")]
+        e.set_field(zero)
+        want = []
+        for _ in got:
+            e.gradient()
+            e.update_avg_weight(20000.0)
+            want.append(e.error())
+        assert len(got) >= 1
+        np.testing.assert_allclose(got, want, rtol=1e-6)
         # omega (alpha + beta / k), k = 1, 2
         r = run(["sirt-mimo-weight", "fo.txt", 160, 0, 2, 1.0, 10000.0, 2.0], tmp_path)
         assert r.returncode == 0, r.stderr

@@ -273,6 +273,25 @@ def test_update_weighted_and_group_match_oracle():
     assert abs(eps - orc.error(e.get_field().astype(np.float64), fo)) < 1e-9
 
 
+def test_update_avg_weight_matches_oracle():
+    """sirt-mimo-avg-weight.cu:80-203 on the engine's own gradients: the
+    distances w_g, their sum and the field after two updates (the second
+    from a non-zero field, which enters the update whole)."""
+    say("update avg-weight")
+    e, c, fo = _engine_with_gradients()
+    f = np.zeros(fo.size, np.float32)
+    for s in (2e3, 5e2):
+        e.set_field(f)
+        e.gradient()
+        G = np.stack([e.get_gradient(g) for g in range(e.num_shots)]).astype(np.float32)
+        w = e.update_avg_weight(s)
+        want, ww = orc.update_avg_weight(c, f.copy(), G, s)
+        np.testing.assert_allclose(w, ww, rtol=1e-12)
+        np.testing.assert_allclose(e.get_field(), want, rtol=2e-6, atol=1e-9)
+        f = e.get_field()
+    assert np.isfinite(f).all()
+
+
 # ----------------------------------------------------- whole iterations
 
 @pytest.mark.parametrize("weighted", [False, True])

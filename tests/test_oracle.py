@@ -255,6 +255,38 @@ def test_update_weighted_closed_form_and_invariants():
     np.testing.assert_allclose(fe[pix[0]], 2.0)
 
 
+def test_update_avg_weight_closed_form():
+    """sirt-mimo-avg-weight.cu:80-203: two groups with single-pixel gradients
+    a e_p, b e_q on a constant field c: both intermediate fields sit
+    s sqrt(a^2 + b^2) / (2 (1 + c)) from their mean, and the whole fields
+    (f included) divided by that distance are added to f."""
+    c = small_ref_cfg(n=16, lo=2)
+    n2 = c.nx * c.ny
+    m = mask_of(c)
+    p, q = 5 + 16 * 5, 9 + 16 * 8
+    out = 0  # a pixel outside the mask
+    assert m[p] and m[q] and not m[out]
+    c0, s, a, b = 0.25, 3.0, 2.0, -1.5
+    G = np.zeros((2, n2))
+    G[0, p], G[1, q] = a, b
+    G[0, out] = 7.0  # outside the mask: no increment, the field still enters
+    f, w = orc.update_avg_weight(c, np.full(n2, c0), G, s)
+    wd = s * np.hypot(a, b) / (2 * (1 + c0))
+    np.testing.assert_allclose(w, [wd, wd, 2 * wd], rtol=1e-14)
+    exp = np.full(n2, c0 + 2 * c0 / wd)
+    exp[p] = c0 + (2 * c0 + s * a / (1 + c0)) / wd
+    exp[q] = c0 + (2 * c0 + s * b / (1 + c0)) / wd
+    np.testing.assert_allclose(f, exp, rtol=1e-14)
+    # permutation symmetry
+    fp, wp = orc.update_avg_weight(c, np.full(n2, c0), G[::-1].copy(), s)
+    np.testing.assert_allclose(fp, f, rtol=1e-14)
+    np.testing.assert_allclose(wp[:2], w[1::-1], rtol=1e-15)
+    # one group: zero distance, non-finite field (as the reference's division by zero)
+    with np.errstate(all="ignore"):
+        f1, w1 = orc.update_avg_weight(c, np.full(n2, c0), G[:1].copy(), s)
+    assert w1[0] == 0.0 and not np.isfinite(f1).any()
+
+
 def test_error_and_schedules():
     fo = orc.phantom_reference()
     f = fo.astype(np.float64).copy()

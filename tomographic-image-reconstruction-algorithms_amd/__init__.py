@@ -113,6 +113,7 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "tomo_update_uniform": (_int, [_vp, ctypes.c_float, _int]),
     "tomo_update_weighted": (_int, [_vp, ctypes.c_float, _vp]),
+    "tomo_update_avg_weight": (_int, [_vp, ctypes.c_float, _vp]),
     "tomo_update_group": (_int, [_vp, _int, ctypes.c_float]),
     "tomo_error": (_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "tomo_sirt_iteration": (_int, [_vp, ctypes.c_float, _int, _int, _vp, _vp]),
@@ -408,6 +409,13 @@ class Engine:
     def update_weighted(self, scale):
         w = np.zeros(self.num_shots, np.float64)
         self._c(lib().tomo_update_weighted(self._ctx, float(scale), _ptr(w)))
+        return w
+
+    def update_avg_weight(self, scale):
+        """Weighted average of intermediate fields (sirt-mimo-avg-weight.cu:
+        80-203); returns [w_1 .. w_Ng, sum w]."""
+        w = np.zeros(self.num_shots + 1, np.float64)
+        self._c(lib().tomo_update_avg_weight(self._ctx, float(scale), _ptr(w)))
         return w
 
     def update_group(self, g, scale):

@@ -1314,6 +1314,25 @@ int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
   return TOMO_OK;
 }
 
+int tomo_update_avg_weight(tomo_ctx *c, float scale, double *weights_out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
+  const int M = c->nshots;
+  const int pb = weighted_part_blocks((int)c->n2);
+  const size_t need = (size_t)c->n2 + (size_t)M * pb + (size_t)M + 1;
+  if ((rc = ensure_scratch(c, need))) return rc;
+  double *mean = c->d_scratch, *part = mean + c->n2, *w = part + (size_t)M * pb;
+  HIPCHK(c, launch_avg_weight(c->d_f, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)scale, mean, part, pb, w,
+                              c->stream));
+  if ((rc = upload_coefficients(c))) return rc;
+  std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
+  if (weights_out)
+    HIPCHK(c, hipMemcpyAsync(weights_out, w, sizeof(double) * (M + 1), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
 int tomo_update_group(tomo_ctx *c, int g, float scale) {
   int rc = check_ctx(c);
   if (rc) return rc;

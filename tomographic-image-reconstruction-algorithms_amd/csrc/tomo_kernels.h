@@ -265,6 +265,11 @@ hipError_t launch_error_quadratic(const float *f, const float *fo, const float *
 // sums {sum (f - fo)^2, sum fo^2} in FP64 into out[2]
 hipError_t launch_error(const float *f, const float *fo, int n, double *out, hipStream_t st);
 
+// weighted average of intermediate fields (sirt-mimo-avg-weight.cu:80-203):
+// mean[n2], part[M * part_blocks], w[M + 1] (w_g and their sum) are scratch
+// the caller provides
+hipError_t launch_avg_weight(float *f, const float *G, int M, int nx, int ny, Mask m, double scale, double *mean,
+                             double *part, int part_blocks, double *w, hipStream_t st);
 // number of partial blocks used by launch_weighted for n2 cells
 int weighted_part_blocks(int n2);
 

@@ -1721,6 +1721,75 @@ __global__ void weighted_update_kernel(float *__restrict__ f, const float *__res
   f[id] = (float)((double)f[id] + (pixw ? d * (double)pixw[id] : d));
 }
 
+// Weighted average of intermediate fields (sirt-mimo-avg-weight.cu:80-203,
+// 530-544) in FP64 sums of fixed order: fg_g = f + scale mask G_g / (1 + f)
+// per group, formed where it is needed and never stored.
+__device__ __forceinline__ double intermediate_field(const float *__restrict__ f, const float *__restrict__ G,
+                                                     long long n2, int g, long long id, bool inside,
+                                                     double scale) {
+  const double fv = (double)f[id];
+  return inside ? fv + scale * ((double)G[g * n2 + id] / (1.0 + fv)) : fv;
+}
+
+// pass 1: the mean of the intermediate fields over the groups
+__global__ void avgw_mean_kernel(const float *__restrict__ f, const float *__restrict__ G, int M, int nx, int ny,
+                                 Mask msk, double scale, double *__restrict__ mean) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long n2 = (long long)nx * ny;
+  if (id >= n2) return;
+  const bool inside = in_mask(msk, (int)(id % nx), (int)(id / nx));
+  double s = 0.0;
+  for (int g = 0; g < M; ++g) s += intermediate_field(f, G, n2, g, id, inside, scale);
+  mean[id] = s / M;
+}
+
+// pass 2: partial squared distances of each intermediate field to the mean,
+// over the full grid
+__global__ __launch_bounds__(256) void avgw_norm_kernel(const float *__restrict__ f, const float *__restrict__ G,
+                                                        const double *__restrict__ mean, int nx, int ny, Mask msk,
+                                                        double scale, int chunk, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int g = blockIdx.y;
+  const long long n2 = (long long)nx * ny;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, n2);
+  double acc = 0.0;
+  for (long long id = beg + threadIdx.x; id < end; id += 256) {
+    const bool inside = in_mask(msk, (int)(id % nx), (int)(id / nx));
+    const double v = intermediate_field(f, G, n2, g, id, inside, scale) - mean[id];
+    acc += v * v;
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) part[(long long)g * gridDim.x + blockIdx.x] = s;
+}
+
+// pass 3: w_g = sqrt(sum of its parts); w[M] = sum_g w_g (get_normalized_weights,
+// get_total_weight)
+__global__ void avgw_weights_kernel(const double *__restrict__ part, int M, int nparts, double *__restrict__ w) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double total = 0.0;
+  for (int g = 0; g < M; ++g) {
+    double d2 = 0.0;
+    for (int b = 0; b < nparts; ++b) d2 += part[(long long)g * nparts + b];
+    w[g] = sqrt(d2);
+    total += w[g];
+  }
+  w[M] = total;
+}
+
+// pass 4: f += sum_g fg_g / w_g (update_weighted_average_field, update_field);
+// every cell, the whole intermediate fields included
+__global__ void avgw_update_kernel(float *__restrict__ f, const float *__restrict__ G, int M, int nx, int ny,
+                                   Mask msk, double scale, const double *__restrict__ w) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long n2 = (long long)nx * ny;
+  if (id >= n2) return;
+  const bool inside = in_mask(msk, (int)(id % nx), (int)(id / nx));
+  double s = 0.0;
+  for (int g = 0; g < M; ++g) s += intermediate_field(f, G, n2, g, id, inside, scale) / w[g];
+  f[id] = (float)((double)f[id] + s);
+}
+
 // Update direction of one unit of relaxation, d = mask * update(s = 1), in
 // FP64: uniform (w == nullptr: sum_g G_g / (1 + f) / div) or weighted
 // (sum_m w_m G_m / (1 + f)); pixw optional
@@ -2045,6 +2114,18 @@ hipError_t launch_weighted(float *f, const float *G, int M, int nx, int ny, Mask
   weighted_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, n2, chunk, part);
   weighted_weights_kernel<<<1, 256, 0, st>>>(part, M, part_blocks, w);
   weighted_update_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, w, pixw);
+  return hipGetLastError();
+}
+
+hipError_t launch_avg_weight(float *f, const float *G, int M, int nx, int ny, Mask m, double scale, double *mean,
+                             double *part, int part_blocks, double *w, hipStream_t st) {
+  const long long n2 = (long long)nx * ny;
+  const int nb = (int)((n2 + 255) / 256);
+  avgw_mean_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, mean);
+  const int chunk = (int)((n2 + part_blocks - 1) / part_blocks);
+  avgw_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, nx, ny, m, scale, chunk, part);
+  avgw_weights_kernel<<<1, 64, 0, st>>>(part, M, part_blocks, w);
+  avgw_update_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, w);
   return hipGetLastError();
 }
 

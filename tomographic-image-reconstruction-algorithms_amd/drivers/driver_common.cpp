@@ -96,8 +96,15 @@ int run_loop(tomo_ctx *ctx, const Loop &L, double (*scale_fn)(int, const void *)
     double misfit = 0, eps = 0;
     int rc;
     const bool art = L.rule == Rule::Art || (L.rule == Rule::ArtThenWeighted && iter < L.art_iterations);
-    if (art) rc = tomo_art_iteration(ctx, scale, &misfit, &eps);
-    else rc = tomo_sirt_iteration(ctx, scale, L.rule != Rule::Uniform, L.mode, &misfit, &eps);
+    if (art) {
+      rc = tomo_art_iteration(ctx, scale, &misfit, &eps);
+    } else if (L.rule == Rule::AvgWeight) {
+      rc = tomo_gradient(ctx, 0, tomo_num_shots(ctx), &misfit);
+      if (!rc) rc = tomo_update_avg_weight(ctx, scale, nullptr);
+      if (!rc) rc = tomo_error(ctx, &eps);
+    } else {
+      rc = tomo_sirt_iteration(ctx, scale, L.rule != Rule::Uniform, L.mode, &misfit, &eps);
+    }
     check(rc, ctx, "iteration");
     curr = (float)eps;
     const float t = std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();

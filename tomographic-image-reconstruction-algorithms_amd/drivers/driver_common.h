@@ -25,8 +25,9 @@ tomo_ctx *reference_context(int group_size, int shots_per_pass, int *ng);
 bool load_inputs(tomo_ctx *ctx, const std::string &fo_filename, int group_size, std::vector<float> &fo);
 
 // Art then Weighted: the first art_iterations iterations are ART, the rest
-// weighted SIRT (sirt-mimo-seq.cu:337-596)
-enum class Rule { Uniform, Weighted, Art, ArtThenWeighted };
+// weighted SIRT (sirt-mimo-seq.cu:337-596); AvgWeight: the weighted average
+// of intermediate fields (sirt-mimo-avg-weight.cu:526-544)
+enum class Rule { Uniform, Weighted, Art, ArtThenWeighted, AvgWeight };
 
 struct Loop {
   Rule rule;

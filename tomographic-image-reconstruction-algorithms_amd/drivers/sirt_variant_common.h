@@ -1,9 +1,10 @@
 // Shared main() of the intermediate-field averaging and group-serial SIRT
-// variants (sirt-mimo-avg.cu, sirt-mimo-weight.cu, sirt-iter.cu,
-// sirt-iter-norm.cu).  Each is a file-driven loop like sirt-mimo-io whose
-// update is one of the engine's:
+// variants (sirt-mimo-avg.cu, sirt-mimo-avg-weight.cu, sirt-mimo-weight.cu,
+// sirt-iter.cu, sirt-iter-norm.cu).  Each is a file-driven loop like
+// sirt-mimo-io whose update is one of the engine's:
 //  - averaging the per-group intermediate fields f + s mask df_g over the
 //    groups equals f + s mask mean_g(df_g), the uniform update in mean mode;
+//  - their distance-weighted sum is tomo_update_avg_weight;
 //  - group-serial gradients (one group at a time) are passes of one group;
 //  - the schedules are omega (alpha + beta / (k+1)^gamma) for fixed gamma.
 // The FP32 atomic sums of the reference become FP64 sums in a fixed order.
@@ -21,7 +22,7 @@ namespace drv {
 
 struct Variant {
   const char *usage;          // argument list after <fo_filename> <sensor group size>
-  Rule rule;                  // Uniform (mean over groups) or Weighted
+  Rule rule;                  // Uniform (mean over groups), Weighted or AvgWeight
   int group_serial;           // 1: passes of one group
   float prev_epsilon_init;    // initial "previous epsilon" of the stop rule
   bool table;                 // iter/epsilon/difference/scale table, else "Iter: / epsilon ="
