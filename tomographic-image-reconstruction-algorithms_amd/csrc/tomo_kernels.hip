@@ -1420,38 +1420,31 @@ __device__ __forceinline__ void side_copies(Row<VEC> &z, int x0, int nx) {
   if (x0 + VEC == nx) z.e[VEC - 1] = z.e[VEC - 2];
 }
 
-// Each lane walks rows r = rs .. qb+1 of its strip: it forms z(k) at row r
-// from the z(k+1) window (rows r-1, r, r+1) and z(k+2) row r, injects the
-// residual at receivers, stores it and correlates it with Lu(k) on the
-// strip's own rows; then it forms z(k-1) at row r-1 from the z(k) window held
-// in registers (rows r-2, r-1, r) and z(k+1) row r-1, and correlates it with
-// Lu(k-1) into the same gradient row, read once and written once.  z(k) is
-// formed on one halo row above and below the strip.  Row 0 of each level is
-// the copy of row 1 and row ny-1 the copy of row ny-2 (backpropagation2), so
-// the window needs nothing from outside the strip's rows and halo; the strip
-// that owns row 1 (ny-2) also writes row 0 (ny-1) and its correlation.  Per two
-// cell-updates: reads z(k+1), z(k+2), u(k), u(k-1), the gradient, writes z(k),
-// z(k-1), the gradient (16 B per cell-update instead of 24).  The arithmetic
-// of every cell is that of adjoint_vec_kernel and adjoint_ring_kernel, so the
-// fields and gradients agree with two one-step launches bit for bit.
-template <int VEC, int RY>
-__global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_eu(2))) void adjoint2_kernel(
-    Adj2Args A) {
+// One strip of a two-step adjoint launch.  Each lane walks the rows of its
+// strip in direction D (+1: down, r = rs .. qb+1; -1: up, r = re .. qa-1):
+// it forms z(k) at row r from the z(k+1) window (rows r-1, r, r+1) and
+// z(k+2) row r, injects the residual at receivers, stores it and correlates
+// it with Lu(k) on the strip's own rows; then it forms z(k-1) at row q = r-D
+// from the z(k) window held in registers (rows q-1, q, q+1) and z(k+1) row
+// q, and correlates it with Lu(k-1) into the same gradient row, read once
+// and written once.  z(k) is formed on one halo row above and below the
+// strip.  Row 0 of each level is the copy of row 1 and row ny-1 the copy of
+// row ny-2 (backpropagation2), so the window needs nothing from outside the
+// strip's rows and halo; the strip that owns row 1 (ny-2) also writes row 0
+// (ny-1) and its correlation, level k before level k-1 in either direction.
+// The windows hold rows behind (r-D) and ahead (r+D); every cell rule takes
+// them as north / south rows in the same operand order whichever way the
+// strip walks, so both directions give the same bits.  Per two cell-updates:
+// reads z(k+1), z(k+2), u(k), u(k-1), the gradient, writes z(k), z(k-1), the
+// gradient (16 B per cell-update instead of 24).  The arithmetic of every
+// cell is that of adjoint_vec_kernel and adjoint_ring_kernel, so the fields
+// and gradients agree with two one-step launches bit for bit.
+template <int VEC, int RY, int D>
+__device__ __forceinline__ void adjoint2_strip(const Adj2Args &A, const float2 *s_sp,
+                                               float (&stash)[VEC][kBlockX], int shot, int y0) {
   const int nx = A.nx, ny = A.ny;
-  __shared__ float2 s_sp[kSpongeLds];
-  // the gradient row after level k, held from one row to the next for level
-  // k-1 in a lane-private LDS slot instead of registers ([wave][cell][lane]:
-  // conflict-free)
-  __shared__ float s_acc[kBlockY][VEC][kBlockX];
-  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY)
-    s_sp[i] = make_float2(A.sp_a[i], A.sp_ab[i]);
-  __syncthreads();
-  const int lb = xcd_logical_block();
-  const int shot = lb / A.blocks_per_shot;
-  const int y0 = ((lb % A.blocks_per_shot) * kBlockY + threadIdx.y) * RY;
   const int yend = min(y0 + RY, ny);
   const int qa = max(y0, 1), qb = min(yend, ny - 1) - 1;  // own rows of the interior rule
-  if (y0 >= ny || qa > qb) return;                          // whole wavefront
   const int x0 = threadIdx.x * VEC;
   const int gs = A.shot0 + shot;
   const long long so = (long long)shot * A.sstride;
@@ -1472,6 +1465,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
   };
   // z(k) by the rule on rows [rs, re]
   const int rs = max(qa - 1, 1), re = min(qb + 1, ny - 2);
+  auto rule = [&](int y) { return y >= rs && y <= re; };
   auto own = [&](int y) { return y >= qa && y <= qb; };
   auto lrow = [&](int y) { return y >= qa - 1 && y <= qb + 1; };  // u rows of the Laplacians
   unsigned long long rflags = 0;                                     // rows rs .. re holding receivers
@@ -1479,68 +1473,83 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
   for (int i = 0; i < RY + 2; ++i)
     if (rs + i <= re && A.row_has_recv[rs + i]) rflags |= 1ull << i;
   auto desc = [&](int y) {
-    const bool on = y >= rs && y <= re && ((rflags >> (y - rs)) & 1ull);
+    const bool on = rule(y) && ((rflags >> (y - rs)) & 1ull);
     return on ? A.recv_seg[y * kBlockX + threadIdx.x] : -1;
   };
   auto resid = [&](int d, const float *__restrict__ r) { return d >= 0 ? r[d / VEC] : 0.f; };
+  // north / south of a row from the windows' rows behind (r-D) and ahead (r+D)
+  auto north = [](const Row<VEC> &behind, const Row<VEC> &ahead) -> const Row<VEC> & {
+    return D > 0 ? behind : ahead;
+  };
+  auto south = [](const Row<VEC> &behind, const Row<VEC> &ahead) -> const Row<VEC> & {
+    return D > 0 ? ahead : behind;
+  };
 
-  Row<VEC> zu = ld(z1, rs - 1, true), zc = ld(z1, rs, true), zn = ld(z1, rs + 1, true);
-  Row<VEC> pp = ld(pw, rs - 2, false), pu = ld(pw, rs - 1, true), pc = ld(pw, rs, true), pn = ld(pw, rs + 1, true);
-  Row<VEC> uu = ld(u0, rs - 1, lrow(rs - 1)), uc = ld(u0, rs, lrow(rs)), un = ld(u0, rs + 1, lrow(rs + 1));
-  Row<VEC> m2 = ld(um, rs - 2, false), m1 = ld(um, rs - 1, lrow(rs - 1)), m0 = ld(um, rs, lrow(rs));
-  Row<VEC> zz = ld(z2, rs, true);
-  Row<VEC> ac = ld(acc, rs, own(rs));
-  Row<VEC> v2 = ld(acc, 0, false), v1 = v2;  // z(k) rows r-2, r-1
-  float(&stash)[VEC][kBlockX] = s_acc[threadIdx.y];
-  int dr = desc(rs), dn = desc(rs + 1), dq = -1;
+  const int r0 = D > 0 ? rs : re, rlast = D > 0 ? qb + 1 : qa - 1;
+  // windows: z(k+1) rows r-D, r, r+D; coefficients r-2D .. r+D; u(k) rows
+  // r-D .. r+D; u(k-1) rows r-2D .. r; z(k+2) and the gradient at row r;
+  // z(k) rows r-2D, r-D before row r is formed
+  Row<VEC> zb = ld(z1, r0 - D, true), zc = ld(z1, r0, true), za = ld(z1, r0 + D, true);
+  Row<VEC> pp = ld(pw, r0 - 2 * D, false), pb = ld(pw, r0 - D, true), pc = ld(pw, r0, true),
+           pa = ld(pw, r0 + D, true);
+  Row<VEC> ub = ld(u0, r0 - D, lrow(r0 - D)), uc = ld(u0, r0, lrow(r0)), ua = ld(u0, r0 + D, lrow(r0 + D));
+  Row<VEC> m2 = ld(um, r0 - 2 * D, false), m1 = ld(um, r0 - D, lrow(r0 - D)), m0 = ld(um, r0, lrow(r0));
+  Row<VEC> zz = ld(z2, r0, true);
+  Row<VEC> ac = ld(acc, r0, own(r0));
+  Row<VEC> v2 = ld(acc, 0, false), v1 = v2;
+  int dr = desc(r0), dn = desc(r0 + D), dq = -1;
   float rk = resid(dr, rrk), rkm = resid(dr, rrm), rq = 0.f;
 
 #pragma unroll 1
-  for (int r = rs; r <= qb + 1; ++r) {
+  for (int r = r0; D > 0 ? r <= rlast : r >= rlast; r += D) {
     // the next row's inputs are in flight while this row computes
-    const Row<VEC> zn2 = ld(z1, r + 2, r + 1 <= re);
-    const Row<VEC> pn2 = ld(pw, r + 2, r + 1 <= re);
-    const Row<VEC> un2 = ld(u0, r + 2, lrow(r + 2));
-    const Row<VEC> mn = ld(um, r + 1, lrow(r + 1));
-    const Row<VEC> zzn = ld(z2, r + 1, r + 1 <= re);
-    const Row<VEC> an = ld(acc, r + 1, own(r + 1));
-    const int d2 = desc(r + 2);
+    const Row<VEC> za2 = ld(z1, r + 2 * D, rule(r + D));
+    const Row<VEC> pa2 = ld(pw, r + 2 * D, rule(r + D));
+    const Row<VEC> ua2 = ld(u0, r + 2 * D, lrow(r + 2 * D));
+    const Row<VEC> mn = ld(um, r + D, lrow(r + D));
+    const Row<VEC> zzn = ld(z2, r + D, rule(r + D));
+    const Row<VEC> an = ld(acc, r + D, own(r + D));
+    const int d2 = desc(r + 2 * D);
     // z(k) at row r
     Row<VEC> v0;
-    if (r <= re) {
-      v0 = adjoint_row<VEC>(s_sp, x0, nx, pu, zu, pn, zn, pc, zc, zz, r, ny, A.sponge_w);
+    if (rule(r)) {
+      v0 = adjoint_row<VEC>(s_sp, x0, nx, north(pb, pa), north(zb, za), south(pb, pa), south(zb, za), pc, zc, zz, r,
+                            ny, A.sponge_w);
       inject<VEC>(v0, dr, rk, A.gain);
       if (own(r)) {
-        ac = correlate_row<VEC>(uu, uc, un, ac, v0, x0, nx, r, ny, A.inv_h2);
+        const Row<VEC> &un = north(ub, ua), &us = south(ub, ua);
+        ac = correlate_row<VEC>(un, uc, us, ac, v0, x0, nx, r, ny, A.inv_h2);
         store_row<VEC>(z0 + r * nx + x0, v0);
         // rows 0 and ny-1 of z(k): copies of rows 1 and ny-2
-        if (r == 1) adjoint_edge_row<VEC, true>(v0, uu, uc, un, z0, acc, x0, nx, A.inv_h2);
+        if (r == 1) adjoint_edge_row<VEC, true>(v0, un, uc, us, z0, acc, x0, nx, A.inv_h2);
         if (r == ny - 2)
-          adjoint_edge_row<VEC, false>(v0, un, uc, uu, z0 + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
+          adjoint_edge_row<VEC, false>(v0, us, uc, un, z0 + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
       } else {
         side_copies<VEC>(v0, x0, nx);
       }
     } else {
-      v0 = v1;  // row ny-1: the copy of row ny-2
+      v0 = v1;  // row ny-1 (0): the copy of row ny-2 (1)
     }
-    if (r == 1) v1 = v0;  // row 0: the copy of row 1
-    // z(k-1) at row q = r-1
-    const int q = r - 1;
+    if (r == (D > 0 ? 1 : ny - 2)) v1 = v0;  // row 0 (ny-1) before the walk reaches it
+    // z(k-1) at row q = r-D
+    const int q = r - D;
     if (own(q)) {
       Row<VEC> aq;
 #pragma unroll
       for (int e = 0; e < VEC; ++e) aq.e[e] = stash[e][threadIdx.x];
-      Row<VEC> w = adjoint_row<VEC>(s_sp, x0, nx, pp, v2, pc, v0, pu, v1, zu, q, ny, A.sponge_w);
+      Row<VEC> w = adjoint_row<VEC>(s_sp, x0, nx, north(pp, pc), north(v2, v0), south(pp, pc), south(v2, v0), pb,
+                                    v1, zb, q, ny, A.sponge_w);
       inject<VEC>(w, dq, rq, A.gain);
-      const Row<VEC> ao = correlate_row<VEC>(m2, m1, m0, aq, w, x0, nx, q, ny, A.inv_h2);
+      const Row<VEC> &mn_ = north(m2, m0), &ms_ = south(m2, m0);
+      const Row<VEC> ao = correlate_row<VEC>(mn_, m1, ms_, aq, w, x0, nx, q, ny, A.inv_h2);
       store_row<VEC>(zm + q * nx + x0, w);
       store_row<VEC>(acc + q * nx + x0, ao);
       // rows 0 and ny-1 of z(k-1)
-      if (q == 1) adjoint_edge_row<VEC, true>(w, m2, m1, m0, zm, acc, x0, nx, A.inv_h2);
+      if (q == 1) adjoint_edge_row<VEC, true>(w, mn_, m1, ms_, zm, acc, x0, nx, A.inv_h2);
       if (q == ny - 2)
-        adjoint_edge_row<VEC, false>(w, m0, m1, m2, zm + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
+        adjoint_edge_row<VEC, false>(w, ms_, m1, mn_, zm + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
     }
-    // residuals of row r+1 (its descriptor was read a row ago)
+    // residuals of row r+D (its descriptor was read a row ago)
     const float rkn = resid(dn, rrk), rkmn = resid(dn, rrm);
     if (own(r)) {
 #pragma unroll
@@ -1549,16 +1558,16 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
     ac = an;
     v2 = v1;
     v1 = v0;
-    pp = pu;
-    pu = pc;
-    pc = pn;
-    pn = pn2;
-    zu = zc;
-    zc = zn;
-    zn = zn2;
-    uu = uc;
-    uc = un;
-    un = un2;
+    pp = pb;
+    pb = pc;
+    pc = pa;
+    pa = pa2;
+    zb = zc;
+    zc = za;
+    za = za2;
+    ub = uc;
+    uc = ua;
+    ua = ua2;
     m2 = m1;
     m1 = m0;
     m0 = mn;
@@ -1570,6 +1579,34 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
     rk = rkn;
     rkm = rkmn;
   }
+}
+
+// Two adjoint steps per launch over strips of RY rows.  Even strips walk
+// down, odd strips up, so two neighbouring strips reach their shared halo
+// rows at the same time (both at the start or both at the end of their
+// walks) and the second read of a row is served by L2.
+template <int VEC, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_eu(2))) void adjoint2_kernel(
+    Adj2Args A) {
+  const int ny = A.ny;
+  __shared__ float2 s_sp[kSpongeLds];
+  // the gradient row after level k, held from one row to the next for level
+  // k-1 in a lane-private LDS slot instead of registers ([wave][cell][lane]:
+  // conflict-free)
+  __shared__ float s_acc[kBlockY][VEC][kBlockX];
+  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY)
+    s_sp[i] = make_float2(A.sp_a[i], A.sp_ab[i]);
+  __syncthreads();
+  const int lb = xcd_logical_block();
+  const int shot = lb / A.blocks_per_shot;
+  const int strip = (lb % A.blocks_per_shot) * kBlockY + threadIdx.y;
+  const int y0 = strip * RY;
+  const int yend = min(y0 + RY, ny);
+  if (y0 >= ny || max(y0, 1) > min(yend, ny - 1) - 1) return;  // whole wavefront
+  if (strip & 1)
+    adjoint2_strip<VEC, RY, -1>(A, s_sp, s_acc[threadIdx.y], shot, y0);
+  else
+    adjoint2_strip<VEC, RY, 1>(A, s_sp, s_acc[threadIdx.y], shot, y0);
 }
 
 
