@@ -81,6 +81,39 @@ def test_many_receivers_on_strip_and_halo_rows(steps):
         e2.close()
 
 
+@pytest.mark.parametrize("ny,rows", [(96, 32), (160, 32), (96, 8)])
+def test_odd_strip_counts_on_rectangular_grids(ny, rows):
+    """Rectangular grids (nx = 128, ny = 96 or 160) with 3, 5 or 12 strips.
+    The three-step forward's last strip forms the bottom edge rows at the end
+    of its walk.  The two-step adjoint's strips alternate their walking
+    direction (even strips down, odd up): with an odd number of strips the
+    last one walks down, with an even number it walks up and writes the
+    bottom edge row at the start."""
+    print(f"[case] rectangular 128x{ny}, rows {rows}", flush=True)
+    engines = []
+    for steps in (1, 3):
+        c = unit_config(128, 241)
+        c.ny = ny
+        c.mask_y0, c.mask_y1 = ny // 4, ny - ny // 4
+        c.checkpoint_every = 40
+        ii, jj = tomo.transducers_ring(128, ny, 8, 0.39 * min(128, ny))
+        e = tomo.Engine(c, device=0)
+        e.set_transducers(ii, jj, one_based=False)
+        e.set_receivers(ii, jj, one_based=False)
+        e.set_groups(1)
+        e.set_shape(4, rows, rows, 1, 1)
+        e.set_time_blocking(steps)
+        engines.append(e)
+    e1, e3 = engines
+    try:
+        assert e1.time_blocking() == 1 and e3.time_blocking() == 3 and e3.adjoint_blocking() == 2
+        compare(e1, e3, c, 8)
+        np.testing.assert_array_equal(e3.forward(fp64=False), e1.forward(fp64=False))
+    finally:
+        e1.close()
+        e3.close()
+
+
 def test_fallback_to_one_step_adjoint():
     print("[case] adjoint fallback: coverage, crowded receivers", flush=True)
     # coverage accumulation: forward pairs, one-step adjoint

@@ -21,6 +21,14 @@ __device__ __forceinline__ int edge_dist(int x, int y, int nx, int ny) {
   return min(min(x, y), min(nx - 1 - x, ny - 1 - y));
 }
 
+// The wavefront's row of the block.  Blocks are kBlockX = 64 lanes wide, one
+// wavefront per threadIdx.y, so the value is the same in every lane; reading
+// it through readfirstlane lets the compiler keep the strip bounds and the
+// row loop in scalar registers.  The two-step adjoint uses it (it frees the
+// VGPRs of a spill there); in the forward kernels it measured slower.
+__device__ __forceinline__ int wave_row() { return __builtin_amdgcn_readfirstlane((int)threadIdx.y); }
+static_assert(kBlockX == 64, "one wavefront per block row");
+
 // ------------------------------------------------------- cell arithmetic
 // Device code is compiled without automatic FMA contraction; every cell rule
 // below is one fixed chain of fused multiply-adds, shared by all kernels that
@@ -1599,7 +1607,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
   __syncthreads();
   const int lb = xcd_logical_block();
   const int shot = lb / A.blocks_per_shot;
-  const int strip = (lb % A.blocks_per_shot) * kBlockY + threadIdx.y;
+  const int strip = (lb % A.blocks_per_shot) * kBlockY + wave_row();
   const int y0 = strip * RY;
   const int yend = min(y0 + RY, ny);
   if (y0 >= ny || max(y0, 1) > min(yend, ny - 1) - 1) return;  // whole wavefront
