@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--c4-threads", type=int, default=4,
+                    help="host threads stepping the C4 phantoms concurrently (1: one after the other)")
     ap.add_argument("--graphs", action="store_true", help="replay each gradient pass as a captured HIP graph")
     ap.add_argument("--study", choices=["c2", "ttt", "train"], default=None,
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
@@ -343,10 +345,13 @@ def main():
 def run_c4(args):
     """Uneven set: 64 phantoms of sides {128..512}, n/4 ring shots, 8n steps,
     stencil shapes from the autotuner per side; a step is one weighted SIRT
-    iteration of every phantom, the phantoms stepped one context after the
-    other (the one-launch-per-level batch of tomo_batch is switched off at
-    this size: its one full-size run ended in a host SIGSEGV whose cause is
-    not established)."""
+    iteration of every phantom.  The phantoms are dealt by cost (n^4) to
+    --c4-threads host threads, each stepping its contexts one after the
+    other on their own streams, so launches of different phantoms overlap
+    on the GPU (the one-launch-per-level batch of tomo_batch is switched off
+    at this size: its one full-size run ended in a host SIGSEGV whose cause
+    is not established)."""
+    from concurrent.futures import ThreadPoolExecutor
     log = (lambda m: print("[bench]", m, file=sys.stderr, flush=True))
     t = time.perf_counter()
     engines = W.c4_problems(log=log)
@@ -362,10 +367,25 @@ def run_c4(args):
         g = np.stack([e.get_gradient(s) for s in range(e.num_shots)]).astype(np.float64).sum(0) / e.num_shots
         scales.append(float(1e-3 / max(np.abs(g).max(), 1e-30)))
 
+    # deal the phantoms to the threads, largest first, each to the least loaded
+    nthreads = max(1, args.c4_threads)
+    groups = [[] for _ in range(nthreads)]
+    load = [0.0] * nthreads
+    for i in sorted(range(len(engines)), key=lambda i: -float(sides[i]) ** 4):
+        t_min = min(range(nthreads), key=lambda t: load[t])
+        groups[t_min].append(i)
+        load[t_min] += float(sides[i]) ** 4
+
+    def step_group(idx):
+        for i in idx:
+            engines[i].gradient()
+            engines[i].update_weighted(scales[i])
+
+    pool = ThreadPoolExecutor(max_workers=nthreads)
+
     def iteration():
-        for e, sc in zip(engines, scales):
-            e.gradient()
-            e.update_weighted(sc)
+        for f in [pool.submit(step_group, g) for g in groups if g]:
+            f.result()
 
     for _ in range(args.warmup):
         iteration()
@@ -376,6 +396,7 @@ def run_c4(args):
         iteration()
         log(f"step {k} done")
     elapsed = time.perf_counter() - t0
+    pool.shutdown()
     cells = 0.0
     fwd_ms = adj_ms = 0.0
     for e in engines:
@@ -395,9 +416,11 @@ def run_c4(args):
         "config": {
             "workload": "uneven set: 64 phantoms, sides from {128,192,256,384,512}, n/4 ring shots, 8n steps, "
                         "checkpoint every 128, passes of 32 shots, one weighted SIRT iteration of every phantom "
-                        "per step, phantoms stepped one after the other",
+                        f"per step, phantoms stepped by {nthreads} host threads on their own streams",
+            "host_threads": nthreads,
             "sides": sides, "shapes": {str(k): list(v) for k, v in shapes.items()},
             "cell_updates_per_step": cells / args.steps, "mean_eps_after_steps": eps,
+            # stencil spans summed over the phantoms (they overlap when threads > 1)
             "stencil_ms_share": round((fwd_ms + adj_ms) / (elapsed * 1e3), 4),
         },
         "roofline": None,

@@ -74,7 +74,8 @@ struct tomo_ctx {
   std::vector<hipEvent_t> ev_free;
   std::vector<TimedSpan> spans;
   double t_ms[3] = {0, 0, 0}, t_launch[3] = {0, 0, 0}, t_cells[3] = {0, 0, 0};
-  // forward steps per launch of the gradient passes (1, or 2 where supported)
+  // most forward steps per launch (1, 2 or 3; fewer where the larger
+  // blockings do not apply)
   int time_block = 3;
   // per-pixel illumination weights of the updates (0 off, 1 coverage)
   int pixw_mode = 0;

@@ -342,9 +342,8 @@ bool two_step_adjoint(const tomo_ctx *c) {
 
 // location of forward level k during the first (recording) pass (the
 // rolling levels use a ring of ring_levels(c))
-float *pass1_level(tomo_ctx *c, int P, int k, bool need_history) {
+float *pass1_level(tomo_ctx *c, int P, int k) {
   const size_t lvl = (size_t)P * c->n2;
-  if (!need_history) return c->d_seg + (size_t)(k % 3) * lvl;
   if (full_history(c)) return c->d_seg + (size_t)k * lvl;
   const int C = c->cfg.checkpoint_every;
   const int q = k % C;
@@ -561,8 +560,9 @@ int forward_traces(tomo_ctx *c, int g0, int ng, T *ring, size_t ring_lvl, const 
   for (int p0 = g0; p0 < g0 + ng; p0 += P) {
     const int np = std::min(P, g0 + ng - p0);
     HIPCHK(c, launch_fill_f32(c->d_rr + (size_t)p0 * tr, (size_t)np * tr, 0.f, c->stream));
-    // FP32 with two steps per launch: a ring of 4 levels (level k+2 is written
-    // while other strips still read level k-1 on their halo rows)
+    // FP32 with m steps per launch: a ring of m + 2 levels (levels k+1 .. k+m
+    // are written while other strips still read levels k-1, k on their halo
+    // rows), as ring_levels
     bool two = false, three = false;
     if constexpr (std::is_same<T, float>::value) {
       two = two_step(c);
@@ -659,8 +659,8 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   const int nt = c->nt;
   HIPCHK(c, launch_fill_f32(c->d_rr + (size_t)p0 * tr, (size_t)np * tr, 0.f, c->stream));
   // pass 1: forward with the residual meas - u recorded at the receivers
-  HIPCHK(c, launch_fill_f32(pass1_level(c, P, 0, true), (size_t)np * c->n2, 0.f, c->stream));
-  HIPCHK(c, launch_fill_f32(pass1_level(c, P, 1, true), (size_t)np * c->n2, 0.f, c->stream));
+  HIPCHK(c, launch_fill_f32(pass1_level(c, P, 0), (size_t)np * c->n2, 0.f, c->stream));
+  HIPCHK(c, launch_fill_f32(pass1_level(c, P, 1), (size_t)np * c->n2, 0.f, c->stream));
   FwdArgs<float> fa = fwd_args<float>(c, p0, c->d_pw, c->d_spa, c->d_spab);
   {
     TimedSpan s;
@@ -672,18 +672,18 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
       if (three && k + 2 < nt - 1) {
         // level k+1 stays in registers unless the history keeps it
         const bool store1 = C == 0 || (k + 1) % C < 2;
-        rc = forward_step3(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
-                           pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true),
-                           pass1_level(c, P, k + 3, true), k, np, true, c->d_meas, c->d_rr, store1);
+        rc = forward_step3(c, fa, pass1_level(c, P, k - 1), pass1_level(c, P, k),
+                           pass1_level(c, P, k + 1), pass1_level(c, P, k + 2),
+                           pass1_level(c, P, k + 3), k, np, true, c->d_meas, c->d_rr, store1);
         k += 3;
       } else if (two && k + 1 < nt - 1) {
-        rc = forward_step2(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
-                           pass1_level(c, P, k + 1, true), pass1_level(c, P, k + 2, true), k, np, true, c->d_meas,
+        rc = forward_step2(c, fa, pass1_level(c, P, k - 1), pass1_level(c, P, k),
+                           pass1_level(c, P, k + 1), pass1_level(c, P, k + 2), k, np, true, c->d_meas,
                            c->d_rr);
         k += 2;
       } else {
-        rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1, true), pass1_level(c, P, k, true),
-                                 pass1_level(c, P, k + 1, true), k, np, true, c->d_meas, c->d_rr);
+        rc = forward_step<float>(c, fa, pass1_level(c, P, k - 1), pass1_level(c, P, k),
+                                 pass1_level(c, P, k + 1), k, np, true, c->d_meas, c->d_rr);
         k += 1;
       }
       if (rc) return rc;
@@ -904,7 +904,7 @@ namespace tomo_internal {
 
 // entry points of the batched stepping (tomo_batch.hip)
 StencilShape context_shape(const tomo_ctx *c) { return stencil_shape(c); }
-float *level_pass1(tomo_ctx *c, int P, int k) { return pass1_level(c, P, k, true); }
+float *level_pass1(tomo_ctx *c, int P, int k) { return pass1_level(c, P, k); }
 float *level_segment(tomo_ctx *c, int P, int m, int k) { return segment_level(c, P, m, k); }
 FwdArgs<float> forward_args(tomo_ctx *c, int shot0) {
   FwdArgs<float> a = fwd_args<float>(c, shot0, c->d_pw, c->d_spa, c->d_spab);
@@ -1590,7 +1590,7 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
   aa.z1 = Z1;
   aa.z2 = Z2;
   aa.z0 = Z0;
-  aa.u = pass1_level(c, P, 0, true);
+  aa.u = pass1_level(c, P, 0);
   aa.level = mid;
   const int reps = std::max(4, std::min(32, (int)(4.0e8 / ((double)lvl + 1.0))));
   TuneEntry best{c->nx, c->ny, P, sponge, cov, StencilShape{}, 1e30, 1e30};
@@ -1645,8 +1645,9 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
       best.adj_ms = ba;
     }
   }
-  // with two steps per launch the one-step kernels only run odd tails: the
-  // strip heights are then those of the two-step kernels (same lane width)
+  // with two or three steps per launch the one-step kernels only run the
+  // tails: the strip heights are then those of the blocked kernels (same
+  // lane width)
   if (best.shape.vec != 0) {
     c->shape_req = best.shape;
     if (two_step(c)) {
@@ -1673,7 +1674,7 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
         // kernels run all but the tails: they are timed instead (a recording
         // launch, level k+1 dropped, and a replay launch storing all three)
         if (!rc && three_step(c)) {
-          float *U3 = pass1_level(c, P, 2, true);
+          float *U3 = pass1_level(c, P, 2);
           double t = 0.0;
           rc = time_launches(c, reps, [&]() -> int {
             int r = forward_step3(c, fa, Z0, Z1, Z2, Z3, U3, mid, P, false, nullptr, nullptr, false);
@@ -1687,8 +1688,8 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
           b.z2 = Z2;
           b.z0 = Z0;
           b.zm = Z3;
-          b.u0 = pass1_level(c, P, 0, true);
-          b.um = pass1_level(c, P, 1, true);
+          b.u0 = pass1_level(c, P, 0);
+          b.um = pass1_level(c, P, 1);
           b.level = mid;
           rc = time_launches(c, reps, [&]() -> int {
             HIPCHK(c, launch_adjoint2(b, P, adjoint2_rows(c), c->stream));
