@@ -345,9 +345,10 @@ int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_d
  * only the last two levels of each launch: 16 bytes per three
  * cell-updates), else two or one.  Where the forward pass runs two or three
  * steps, the adjoint sweep runs two (16 instead of 24 bytes per
- * cell-update) when the receivers are points with additive injection, no
- * 64th of a row holds two of them, and coverage is off.  Results are the
- * same bit for bit.  tomo_get_time_blocking returns the forward steps per
+ * cell-update) when the receivers are points with additive injection and no
+ * 64th of a row holds two of them; the coverage map (tomo_enable_coverage)
+ * is then summed per checkpoint segment by a pass of its own, in the
+ * one-step sweep's order.  Results are the same bit for bit.  tomo_get_time_blocking returns the forward steps per
  * launch in effect, tomo_get_adjoint_blocking the adjoint's.  No reference
  * counterpart (one launch per step, sirt-mimo.cu:291-299). */
 int tomo_set_time_blocking(tomo_ctx *ctx, int steps);

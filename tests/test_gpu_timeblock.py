@@ -5,9 +5,10 @@ each full-row lane width (nx = 128, 256, 512 -> 2, 4, 8 cells per lane), with
 an odd number of steps (a single-step tail), checkpoint segments of odd and
 even length (adjoint pairs that stop short of a segment's first level), the
 edge/receiver passes around each pair, and receivers on strip and halo
-rows.  Where the two-step adjoint does not apply (coverage on, two receivers
-in one lane segment) the engine falls back to one-step adjoint launches with
-the same results."""
+rows.  Where the two-step adjoint does not apply (two receivers in one lane
+segment) the engine falls back to one-step adjoint launches with
+the same results; with coverage on, a pass per checkpoint segment sums the
+coverage map in the one-step sweep's order."""
 import numpy as np
 import pytest
 
@@ -114,17 +115,28 @@ def test_odd_strip_counts_on_rectangular_grids(ny, rows):
         e3.close()
 
 
-def test_fallback_to_one_step_adjoint():
-    print("[case] adjoint fallback: coverage, crowded receivers", flush=True)
-    # coverage accumulation: forward pairs, one-step adjoint
-    e1, c = engine(256, 201, 40, 1, coverage=True)
-    e2, _ = engine(256, 201, 40, 2, coverage=True)
+@pytest.mark.parametrize("n,nt,ckpt,steps", [(256, 201, 40, 2), (256, 201, 40, 3), (128, 160, 7, 3),
+                                             (128, 121, 0, 3)])
+def test_coverage_with_blocked_adjoint(n, nt, ckpt, steps):
+    """Coverage on: the two-step adjoint carries none, a pass per checkpoint
+    segment (or over the full history) sums u(k)^2 newest level first, corner
+    cells as the mean of their edge neighbours -- the values and order of the
+    one-step sweep, so the coverage maps agree bit for bit."""
+    print(f"[case] coverage, blocking {steps}, n={n} nt={nt} ckpt={ckpt}", flush=True)
+    e1, c = engine(n, nt, ckpt, 1, coverage=True)
+    e2, _ = engine(n, nt, ckpt, steps, coverage=True)
     try:
-        assert e2.time_blocking() == 2 and e2.adjoint_blocking() == 1
+        assert e1.adjoint_blocking() == 1
+        assert e2.time_blocking() == steps and e2.adjoint_blocking() == 2
         compare(e1, e2, c, 6, coverage=True)
+        assert e2.get_coverage().max() > 0
     finally:
         e1.close()
         e2.close()
+
+
+def test_fallback_to_one_step_adjoint():
+    print("[case] adjoint fallback: crowded receivers", flush=True)
     # two receivers in one 4-cell lane segment of a row: records in passes
     # after the blocked forward launches, one-step adjoint
     ri = np.array([100, 101, 150], np.int32)

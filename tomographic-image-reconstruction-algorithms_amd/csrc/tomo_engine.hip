@@ -332,12 +332,13 @@ bool three_step(const tomo_ctx *c) {
 // the ring holds m + 2 levels
 int ring_levels(const tomo_ctx *c) { return three_step(c) ? 5 : two_step(c) ? 4 : 3; }
 
-// two adjoint steps per launch: with two forward steps per launch, point
-// receivers with additive injection, at most one receiver per lane segment,
-// and no coverage accumulation
+// two adjoint steps per launch: with two or three forward steps per launch,
+// point receivers with additive injection and at most one receiver per lane
+// segment (coverage, when on, is summed per checkpoint segment by its own
+// kernel)
 bool two_step_adjoint(const tomo_ctx *c) {
   return two_step(c) && adjoint2_supported(c->nx, c->cfg.sponge_width) && c->recvseg_ok &&
-         c->cfg.recv_mode == TOMO_RECV_POINTS && !c->coverage;
+         c->cfg.recv_mode == TOMO_RECV_POINTS;
 }
 
 // location of forward level k during the first (recording) pass (the
@@ -707,6 +708,10 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   ash.rows = ash.adj_rows;
   ash.depth = ash.adj_depth;
   const bool two_adj = two_step_adjoint(c);
+  // with two-step adjoint launches the coverage is summed per segment by its
+  // own kernel, so the one-step tails of those passes carry none
+  const bool seg_cov = two_adj && c->coverage;
+  if (seg_cov) aa.cov = nullptr;
   for (int m = nseg - 1; m >= 0; --m) {
     const int k0 = m * C;
     const int cnt = std::min(C, nt - k0);
@@ -773,6 +778,24 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
     s.launches = launches;
     s.cells = (double)launches * np * c->n2;
     span_end(c, s);
+    if (seg_cov) {  // this segment's levels, newest first, as the one-step sweep adds them
+      CovSegArgs cv{};
+      cv.lev01[0] = segment_level(c, P, m, k0);
+      cv.lev01[1] = cnt > 1 ? segment_level(c, P, m, k0 + 1) : cv.lev01[0];
+      cv.seg = c->d_seg;
+      cv.lstride = (long long)lvl;
+      cv.sstride = (long long)c->n2;
+      cv.cov = c->d_cov;
+      cv.nx = c->nx;
+      cv.ny = c->ny;
+      cv.n2 = (int)c->n2;
+      cv.nshots = np;
+      cv.shot0 = p0;
+      cv.k0 = k0;
+      cv.klo = std::max(k0, 1);
+      cv.khi = std::min(k0 + cnt - 1, nt - 2);
+      HIPCHK(c, launch_coverage_segment(cv, c->stream));
+    }
   }
   return TOMO_OK;
 }

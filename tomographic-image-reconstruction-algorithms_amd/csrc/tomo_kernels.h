@@ -204,6 +204,19 @@ struct Adj2Args {
   int sponge_w;
   const float *sp_a, *sp_ab;
 };
+// Coverage of one checkpoint segment's forward levels k = khi .. klo (newest
+// first) into cov[shot0 + s] for the nshots local shots; level k is
+// lev01[k - k0] for k - k0 < 2, else seg + (k - k0) * lstride; shots are
+// sstride apart.  Needs nx % 4 == 0.
+struct CovSegArgs {
+  const float *lev01[2];
+  const float *seg;
+  long long lstride, sstride;
+  float *cov;
+  int nx, ny, n2, nshots, shot0;
+  int k0, klo, khi;
+};
+hipError_t launch_coverage_segment(const CovSegArgs &a, hipStream_t st);
 // nx for which the two-step adjoint exists (vec = nx / 64)
 bool adjoint2_supported(int nx, int sponge_w);
 hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st);

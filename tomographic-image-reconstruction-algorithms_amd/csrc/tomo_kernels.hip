@@ -1618,6 +1618,46 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
 }
 
 
+// Coverage of the forward levels of one checkpoint segment, for passes whose
+// adjoint runs two steps per launch (which carry no coverage):
+// cov(x) = fma(v, v, cov(x)) over the segment's levels k = khi .. klo, newest
+// first, with v = u(k) at the cell and, at the four corner cells, the mean
+// of their two edge neighbours -- the order and the values of the one-step
+// adjoint sweep (adjoint_vec_kernel, adjoint_ring_kernel), so the sums are
+// the same bits.  Level k sits at lev01[k - k0] (k - k0 < 2, the checkpoint
+// pair) or seg + (k - k0) * lstride; each thread owns 4 consecutive cells of
+// one shot and keeps their sums in registers across the levels.
+__global__ __launch_bounds__(256) void coverage_segment_kernel(CovSegArgs A) {
+  const long long n4 = (long long)A.n2 / 4;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= n4 * A.nshots) return;
+  const int shot = (int)(t / n4);
+  const int id = (int)(t % n4) * 4;
+  const int nx = A.nx, ny = A.ny;
+  const long long so = (long long)shot * A.sstride;
+  float *__restrict__ cov = A.cov + (long long)(A.shot0 + shot) * A.n2 + id;
+  float4 c = *reinterpret_cast<const float4 *>(cov);
+  // corner cells among these four: x = 0 or nx-1 on rows 0 / ny-1
+  const int y = id / nx, x = id % nx;
+  const bool edge_row = y == 0 || y == ny - 1;
+  const int yi = y == 0 ? 1 : ny - 2;
+#pragma unroll 1
+  for (int k = A.khi; k >= A.klo; --k) {
+    const int q = k - A.k0;
+    const float *__restrict__ u = (q < 2 ? A.lev01[q] : A.seg + (long long)q * A.lstride) + so;
+    float4 v = *reinterpret_cast<const float4 *>(u + id);
+    if (edge_row) {
+      if (x == 0) v.x = 0.5f * (u[yi * nx] + u[y * nx + 1]);
+      if (x + 3 == nx - 1) v.w = 0.5f * (u[yi * nx + nx - 1] + u[y * nx + nx - 2]);
+    }
+    c.x = fused(v.x, v.x, c.x);
+    c.y = fused(v.y, v.y, c.y);
+    c.z = fused(v.z, v.z, c.z);
+    c.w = fused(v.w, v.w, c.w);
+  }
+  *reinterpret_cast<float4 *>(cov) = c;
+}
+
 // ------------------------------------------------------------- elementwise
 
 template <typename T>
@@ -2171,6 +2211,14 @@ hipError_t launch_avg_weight(float *f, const float *G, int M, int nx, int ny, Ma
   avgw_norm_kernel<<<dim3(part_blocks, M), 256, 0, st>>>(f, G, mean, nx, ny, m, scale, chunk, part);
   avgw_weights_kernel<<<1, 64, 0, st>>>(part, M, part_blocks, w);
   avgw_update_kernel<<<nb, 256, 0, st>>>(f, G, M, nx, ny, m, scale, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_coverage_segment(const CovSegArgs &a, hipStream_t st) {
+  if (a.n2 % 4 != 0 || a.nx % 4 != 0) return hipErrorInvalidValue;
+  if (a.khi < a.klo) return hipSuccess;
+  const long long threads = (long long)a.n2 / 4 * a.nshots;
+  coverage_segment_kernel<<<dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st>>>(a);
   return hipGetLastError();
 }
 
