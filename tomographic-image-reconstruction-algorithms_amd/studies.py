@@ -117,6 +117,9 @@ def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = 0.1, 
     scale = L.calibrate_scale(p, frac)
     t_cal = time.perf_counter() - t
     p.reset()
+    # plain SIRT needs no coverage map: it runs without one, the learned
+    # iterations (whose network reads it) with
+    p.engine.enable_coverage(False)
     tp = L.run_plain(p, scale, plain_iters, stop_on_rise=True)
     rose = len(tp.eps) > 1 and tp.eps[-1] > tp.eps[-2]
     k_stop = len(tp.eps) - 2 if rose else len(tp.eps) - 1
@@ -125,11 +128,13 @@ def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = 0.1, 
         log(f"plain: stops after {len(tp.eps)} iterations ({'error rose' if rose else 'iteration cap'}), "
             f"target eps {target:.3f} at {t_plain:.1f} s")
     p.reset()
+    p.engine.enable_coverage(True)
     t0 = time.perf_counter()
     tl = L.run_learned(net, p, scale, learned_iters, target=target)
     eps, secs, acc = list(tl.eps), list(tl.seconds), list(tl.accepted)
     if eps[-1] > target and len(eps) < plain_iters:
         base = time.perf_counter() - t0
+        p.engine.enable_coverage(False)
         tq = L.run_plain(p, scale, plain_iters - len(eps), target=target)
         eps += tq.eps
         secs += [base + s for s in tq.seconds]
