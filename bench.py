@@ -55,7 +55,7 @@ ADJ_BYTES = {1: 24.0, 2: 16.0}
 # forward cell-update reads u(k), u(k-1) and the per-cell coefficient and
 # writes u(k+1) (16 B); an adjoint cell-update fused with the correlation
 # reads z(k+1), z(k+2), the coefficient and u(k) and writes z(k), with the
-# gradient in registers (20 B).  With two steps per launch these exceed what
+# gradient in registers (20 B).  With several steps per launch these exceed what
 # the kernels must move.
 FWD_UNIT_BYTES = 16.0
 ADJ_UNIT_BYTES = 20.0
