@@ -220,8 +220,6 @@ hipError_t launch_coverage_segment(const CovSegArgs &a, hipStream_t st);
 // nx for which the two-step adjoint exists (vec = nx / 64)
 bool adjoint2_supported(int nx, int sponge_w);
 hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st);
-// the edge/receiver pass alone over a.ring[0 .. a.nring) (grid: cells x shots)
-hipError_t launch_forward_ring(const FwdArgs<float> &a, int nshots, hipStream_t st);
 
 // Batched stencil launches (vectorised kernels, sponge preset): the main
 // kernel over blocks_main blocks (item p: start_main[p] ..), then / before it

@@ -2397,11 +2397,4 @@ hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_forward_ring(const FwdArgs<float> &a, int nshots, hipStream_t st) {
-  if (a.nring <= 0) return hipSuccess;
-  const Launch<FwdArgs<float>> L{a, nullptr, nullptr, 0};
-  forward_ring_kernel<true, false><<<dim3((a.nring + 255) / 256, nshots, 1), 256, 0, st>>>(L);
-  return hipGetLastError();
-}
-
 }  // namespace tomo
