@@ -114,6 +114,18 @@ def load_traffic(kernel, shape, tb):
     return None
 
 
+def cpu_model():
+    """The host CPU's model name (Linux), or None."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def counter_pass(args):
     """One FP32 forward (data) and one gradient (or, --forward-only, the
     recording forward pass of one) of the C3 workload, then a 1 GiB device
@@ -328,7 +340,9 @@ def main():
             "value": round(cpu_cells / t_cpu / 1e9, 4), "unit": "Gcell-updates/s", "cores": orc.num_threads(),
             "kind": "port",
             "sample": f"oracle FP32 gradient (forward + checkpoint recompute + adjoint) of {sample} of the "
-                      f"{shots} shots, {n}x{n}, {steps} steps: {t_cpu:.2f} s",
+                      f"{shots} shots, {n}x{n}, {steps} steps: {t_cpu:.2f} s (the whole iteration, "
+                      f"{shots / sample:g}x the shots, would take {t_cpu * shots / sample:.0f} s at this rate)",
+            "cpu_model": cpu_model(),
         }
     e.close()
     if args.copy_calibration:
