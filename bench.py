@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--study", choices=["c2", "ttt", "train"], default=None,
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
     ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
-    ap.add_argument("--out", default="gpurun_out/preconditioner.npy", help="parameters written by --study train")
+    ap.add_argument("--out", default="preconditioner_trained.npy", help="parameters written by --study train")
     ap.add_argument("--phantoms", type=int, default=48, help="training phantoms of --study train")
     ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of --study train")
     ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
