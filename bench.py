@@ -174,7 +174,7 @@ def main():
     ap.add_argument("--c4-threads", type=int, default=4,
                     help="host threads stepping the C4 phantoms concurrently (1: one after the other)")
     ap.add_argument("--graphs", action="store_true", help="replay each gradient pass as a captured HIP graph")
-    ap.add_argument("--study", choices=["c2", "ttt", "train"], default=None,
+    ap.add_argument("--study", choices=["c2", "ttt", "train", "finetune"], default=None,
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
     ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
     ap.add_argument("--out", default="preconditioner_trained.npy", help="parameters written by --study train")
@@ -213,6 +213,9 @@ def main():
             out = S.c2_study(log=log)
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps, log=log)
+        elif args.study == "finetune":
+            out = S.finetune_study(args.out, base=args.weights, phantoms=args.phantoms,
+                                   train_steps=args.train_steps, log=log)
         else:
             runs = [S.ttt_study(seed=int(v), weights=args.weights, log=log) for v in args.seeds.split(",")]
             sp = [r["speedup"] for r in runs if r["speedup"] is not None]

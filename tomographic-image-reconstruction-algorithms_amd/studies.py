@@ -24,6 +24,9 @@
   misfit-optimal first step, every iterate a sample whose target is the
   trajectory's best iterate (lowest error), 4000 Adam steps of batch 8 at
   learning rate 1e-3.
+- ``finetune``: continues training a network on plain-SIRT trajectories at
+  the evaluation size, 512^2 with 128 transducers and 4000 steps (seeds from
+  8000, disjoint from the held-out 9100-9107), targets as in ``train``.
 """
 from __future__ import annotations
 
@@ -102,6 +105,25 @@ def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, log=None)
     return {"study": "train", "out": out, "phantoms": rep.phantoms, "samples": rep.samples,
             "loss_first": rep.losses[0], "loss_last50": float(np.mean(rep.losses[-50:])),
             "seconds": {k: round(v, 2) for k, v in rep.seconds.items()}, "total_seconds": round(time.perf_counter() - t, 2)}
+
+
+def finetune_study(out: str, base: str | None = None, phantoms: int = 12, iters: int = 20,
+                   train_steps: int = 2000, lr: float = 3e-4, log=None) -> dict:
+    """Continue training the network of `base` (default: the shipped one) on
+    `phantoms` seeded 512^2 phantoms (seeds from 8000) x `iters` plain-SIRT
+    iterations; writes the parameters to `out`."""
+    n = 512
+    net = Preconditioner(device=0, seed=1)
+    net.set_params(np.load(base or WEIGHTS, allow_pickle=False))
+    t = time.perf_counter()
+    rep = L.train(net, n, range(8000, 8000 + phantoms), iters, steps=4000, shots=128, frac=0.1, rounds=0,
+                  train_steps=train_steps, batch=4, lr=lr, target="best", log=log)
+    np.save(out, net.get_params(), allow_pickle=False)
+    net.close()
+    return {"study": "finetune", "out": out, "base": base or WEIGHTS, "phantoms": rep.phantoms,
+            "samples": rep.samples, "loss_first": rep.losses[0], "loss_last50": float(np.mean(rep.losses[-50:])),
+            "seconds": {k: round(v, 2) for k, v in rep.seconds.items()},
+            "total_seconds": round(time.perf_counter() - t, 2)}
 
 
 def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = 0.1, seed: int = 9100,
