@@ -338,8 +338,9 @@ int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_d
 /* Stencil steps per launch in the forward passes: 2 runs two leapfrog
  * steps in one pass over each strip (the intermediate level kept in
  * registers: 8 instead of 12 bytes of traffic per forward cell-update) where
- * it applies -- the sponge preset, vectorised kernels, nx of 128, 256 or
- * 512, checkpoint interval >= 4 -- and 1 otherwise.  3 (default) runs three
+ * it applies -- the sponge preset, vectorised kernels, nx of 128, 192, 256,
+ * 384 or 512 (64 lanes of 2, 3, 4, 6 or 8 cells), checkpoint interval >= 4
+ * -- and 1 otherwise.  3 (default) runs three
  * forward steps per launch where, in addition, the strips tile the rows
  * exactly and the checkpoint interval is >= 5 (a recording pass then writes
  * only the last two levels of each launch: 16 bytes per three

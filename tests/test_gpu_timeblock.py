@@ -1,7 +1,8 @@
 """Two and three stencil steps per launch (the intermediate levels held in
 registers; two for the adjoint), against one step per launch: traces, misfits and
 gradients must be bitwise equal -- the per-cell arithmetic is the same -- for
-each full-row lane width (nx = 128, 256, 512 -> 2, 4, 8 cells per lane), with
+each full-row lane width (nx = 128, 192, 256, 384, 512 -> 2, 3, 4, 6, 8 cells
+per lane; 3 and 6 cells move as single floats and float pairs), with
 an odd number of steps (a single-step tail), checkpoint segments of odd and
 even length (adjoint pairs that stop short of a segment's first level), the
 edge/receiver passes around each pair, and receivers on strip and halo
@@ -51,7 +52,8 @@ def compare(e1, e2, c, shots, coverage=False):
 
 
 @pytest.mark.parametrize("steps", [2, 3])
-@pytest.mark.parametrize("n,nt,ckpt", [(128, 301, 0), (256, 301, 49), (512, 203, 50), (512, 160, 7)])
+@pytest.mark.parametrize("n,nt,ckpt", [(128, 301, 0), (256, 301, 49), (512, 203, 50), (512, 160, 7),
+                                       (192, 241, 40), (384, 203, 50)])
 def test_blocked_launches_match_single_steps(n, nt, ckpt, steps):
     print(f"[case] time blocking {steps} n={n} nt={nt} ckpt={ckpt}", flush=True)
     e1, c = engine(n, nt, ckpt, 1)

@@ -427,15 +427,32 @@ __device__ __forceinline__ void sponge_at(const LaneSponge<VEC> &s, int e, int y
   }
 }
 
+// Widest access that divides a lane segment of VEC cells and keeps its
+// alignment: the whole segment for VEC = 2, 4, 8 (16- or 32-byte aligned), 2
+// cells for VEC = 6 (segments start at 24-byte multiples), 1 cell for VEC = 3.
+template <int VEC>
+struct RowAccess {
+  static constexpr bool kPow2 = (VEC & (VEC - 1)) == 0;
+  static constexpr int kChunk = kPow2 ? VEC : (VEC % 2 == 0 ? 2 : 1);
+};
+
 template <int VEC>
 __device__ __forceinline__ Row<VEC> load_row(const float *__restrict__ p, bool ok) {
-  typedef float vt __attribute__((ext_vector_type(VEC)));
+  constexpr int C = RowAccess<VEC>::kChunk;
+  typedef float vt __attribute__((ext_vector_type(C)));
   Row<VEC> r;
   r.hl = r.hr = 0.f;
   if (ok) {
-    const vt v = *reinterpret_cast<const vt *>(p);
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) r.e[e] = v[e];
+    for (int c = 0; c < VEC / C; ++c) {
+      if constexpr (C == 1) {
+        r.e[c] = p[c];
+      } else {
+        const vt v = *reinterpret_cast<const vt *>(p + c * C);
+#pragma unroll
+        for (int e = 0; e < C; ++e) r.e[c * C + e] = v[e];
+      }
+    }
   } else {
 #pragma unroll
     for (int e = 0; e < VEC; ++e) r.e[e] = 0.f;
@@ -477,11 +494,19 @@ constexpr int kSpongeLds = 256;
 
 template <int VEC>
 __device__ __forceinline__ void store_row(float *__restrict__ p, const Row<VEC> &r) {
-  typedef float vt __attribute__((ext_vector_type(VEC)));
-  vt v;
+  constexpr int C = RowAccess<VEC>::kChunk;
+  typedef float vt __attribute__((ext_vector_type(C)));
 #pragma unroll
-  for (int e = 0; e < VEC; ++e) v[e] = r.e[e];
-  *reinterpret_cast<vt *>(p) = v;
+  for (int c = 0; c < VEC / C; ++c) {
+    if constexpr (C == 1) {
+      p[c] = r.e[c];
+    } else {
+      vt v;
+#pragma unroll
+      for (int e = 0; e < C; ++e) v[e] = r.e[c * C + e];
+      *reinterpret_cast<vt *>(p + c * C) = v;
+    }
+  }
 }
 
 template <int VEC>
@@ -2293,7 +2318,7 @@ hipError_t launch_weights_only(const float *f, const float *G, int M, int nx, in
 }
 
 bool forward2_supported(int nx, int sponge_w) {
-  return sponge_w > 0 && (nx == 128 || nx == 256 || nx == 512);
+  return sponge_w > 0 && (nx == 128 || nx == 192 || nx == 256 || nx == 384 || nx == 512);
 }
 
 hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st) {
@@ -2321,6 +2346,10 @@ hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStrea
     TOMO_FWD2(4)
   } else if (vec == 2) {
     TOMO_FWD2(2)
+  } else if (vec == 6) {
+    TOMO_FWD2(6)
+  } else if (vec == 3) {
+    TOMO_FWD2(3)
   } else {
     return hipErrorInvalidValue;
   }
@@ -2356,6 +2385,10 @@ hipError_t launch_forward3(Fwd3Args a, int nshots, int rows, hipStream_t st) {
     TOMO_FWD3(4)
   } else if (vec == 2) {
     TOMO_FWD3(2)
+  } else if (vec == 6) {
+    TOMO_FWD3(6)
+  } else if (vec == 3) {
+    TOMO_FWD3(3)
   } else {
     return hipErrorInvalidValue;
   }
@@ -2390,6 +2423,10 @@ hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st) {
     TOMO_ADJ2(4)
   } else if (vec == 2) {
     TOMO_ADJ2(2)
+  } else if (vec == 6) {
+    TOMO_ADJ2(6)
+  } else if (vec == 3) {
+    TOMO_ADJ2(3)
   } else {
     return hipErrorInvalidValue;
   }
