@@ -245,8 +245,16 @@ def main():
     t = time.perf_counter()
     meas = e.forward(fp64=args.data == "fp64")
     t_data = time.perf_counter() - t
-    e.set_field(np.zeros(fo.size, np.float32))
+    # the boundary takes host arrays: the time to hand over this problem's
+    # inputs (field and measured traces, host to HBM) is reported beside the
+    # rate, which is measured with them resident
+    meas = np.ascontiguousarray(meas, np.float32)
+    f0 = np.zeros(fo.size, np.float32)
+    t = time.perf_counter()
+    e.set_field(f0)
     e.set_measured(meas)
+    t_upload = time.perf_counter() - t
+    upload_bytes = meas.nbytes + f0.nbytes
     print(f"[bench] workload {args.workload}: {n}x{n}, {shots} shots, {steps} steps; "
           f"{args.data.upper()} data in {t_data:.2f} s", file=sys.stderr, flush=True)
 
@@ -309,6 +317,11 @@ def main():
             "cell_updates_per_step": cells / args.steps, "stencil_shape": list(shape),
             "forward_steps_per_launch": tb, "adjoint_steps_per_launch": ab,
             "eps_after_steps": eps[-1] if eps else None,
+            # host-to-HBM hand-over of the inputs (once per problem), and the
+            # rate of one iteration that includes it
+            "input_upload": {"bytes": upload_bytes, "ms": round(t_upload * 1e3, 3),
+                             "value_with_one_upload": round(cells / args.steps / (elapsed / args.steps + t_upload)
+                                                            / 1e9, 3)},
         },
         "roofline": None,
         "forward": {
