@@ -267,25 +267,42 @@ __global__ __launch_bounds__(256) void wgrad_kernel(WgradArgs a) {
       for (int r = 0; r < 4; ++r) dst[(size_t)(mb * 16 + 4 * q + r) * S::NCOL + nb * 16 + col] = acc[mb][nb][r];
 }
 
-// grad (canonical layout of layer l) = sum over waves of the partials, FP64
+// grad (canonical layout of layer l) = sum over waves of the partials, FP64,
+// in a fixed order: kWredGroups consecutive runs of waves are summed in
+// parallel (pass 1, into red[group][i]), then the runs in order (pass 2)
+constexpr int kWredGroups = 32;
+
 template <int CIN, int COUT>
-__global__ void wgrad_reduce_kernel(const float *part, int nwaves, int cin_real, int cout_real, float *grad) {
+__device__ __forceinline__ size_t wgrad_part_index(int i, int cin_real, int cout_real) {
   using S = WgradShape<CIN, COUT>;
   const int nw = cout_real * cin_real * 9;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nw + cout_real) return;
-  int row, colm;
   if (i < nw) {
     const int o = i / (cin_real * 9), rem = i % (cin_real * 9);
     const int c = rem / 9, tap = rem % 9;
-    row = o;
-    colm = tap * CIN + c;
-  } else {
-    row = i - nw;
-    colm = S::NREAL;
+    return (size_t)o * S::NCOL + tap * CIN + c;
   }
+  return (size_t)(i - nw) * S::NCOL + S::NREAL;
+}
+
+template <int CIN, int COUT>
+__global__ void wgrad_reduce_kernel(const float *part, int nwaves, int cin_real, int cout_real, double *red) {
+  using S = WgradShape<CIN, COUT>;
+  const int total = cout_real * cin_real * 9 + cout_real;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (i >= total) return;
+  const size_t at = wgrad_part_index<CIN, COUT>(i, cin_real, cout_real);
+  const int run = (nwaves + kWredGroups - 1) / kWredGroups;
+  const int w1 = min(nwaves, (g + 1) * run);
   double s = 0.0;
-  for (int w = 0; w < nwaves; ++w) s += (double)part[((size_t)w * S::MROW + row) * S::NCOL + colm];
+  for (int w = g * run; w < w1; ++w) s += (double)part[(size_t)w * S::MROW * S::NCOL + at];
+  red[(size_t)g * total + i] = s;
+}
+
+__global__ void wgrad_finish_kernel(const double *red, int total, float *grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  double s = 0.0;
+  for (int g = 0; g < kWredGroups; ++g) s += red[(size_t)g * total + i];
   grad[i] = (float)s;
 }
 
@@ -515,6 +532,7 @@ struct tomo_cnn {
   __bf16 *d_g8 = nullptr;               // output-layer gradient [p][8]
   __bf16 *d_ga = nullptr, *d_gb = nullptr;  // hidden gradients [p][32] (ping-pong)
   float *d_part = nullptr;              // weight-gradient partials
+  double *d_wred = nullptr;             // their sums over runs of waves
   size_t part_n = 0;
   double *d_red = nullptr;              // reductions (loss partials, stats)
   float *d_tmp3 = nullptr;              // host-staged input [p][3] / target / dy
@@ -589,6 +607,7 @@ int ensure_batch(tomo_cnn *n, size_t npix) {
   const int nw = wgrad_waves(npix);
   const size_t part = std::max(wgrad_part_elems<32, 32>(nw), wgrad_part_elems<8, 32>(nw));
   if ((rc = nalloc(n, &n->d_part, std::max(part, wgrad_part_elems<32, 8>(nw))))) return rc;
+  if (!n->d_wred && (rc = nalloc(n, &n->d_wred, (size_t)kWredGroups * (32 * 32 * 9 + 32)))) return rc;
   n->part_n = part;
   n->cap_pix = npix;
   return TOMO_OK;
@@ -648,8 +667,11 @@ int launch_wgrad(tomo_cnn *n, const __bf16 *in, const __bf16 *d, int layer, int 
   hipLaunchKernelGGL((wgrad_kernel<CIN, COUT>), dim3(nw / kWgWaves), dim3(64 * kWgWaves), 0, n->stream, a);
   CNNCHK(n, hipGetLastError());
   const int total = kCout[layer] * kCin[layer] * 9 + kCout[layer];
-  hipLaunchKernelGGL((wgrad_reduce_kernel<CIN, COUT>), dim3(blocks_for(total)), dim3(256), 0, n->stream,
-                     (const float *)n->d_part, nw, kCin[layer], kCout[layer], n->d_grad + param_offset(layer));
+  hipLaunchKernelGGL((wgrad_reduce_kernel<CIN, COUT>), dim3(blocks_for(total), kWredGroups), dim3(256), 0,
+                     n->stream, (const float *)n->d_part, nw, kCin[layer], kCout[layer], n->d_wred);
+  CNNCHK(n, hipGetLastError());
+  hipLaunchKernelGGL(wgrad_finish_kernel, dim3(blocks_for(total)), dim3(256), 0, n->stream,
+                     (const double *)n->d_wred, total, n->d_grad + param_offset(layer));
   CNNCHK(n, hipGetLastError());
   return TOMO_OK;
 }
@@ -804,7 +826,8 @@ void tomo_cnn_destroy(tomo_cnn *n) {
   };
   fr(n->d_params); fr(n->d_grad); fr(n->d_m); fr(n->d_v); fr(n->d_red);
   for (int l = 0; l < kLayers; ++l) { fr(n->d_wf[l]); fr(n->d_wb[l]); fr(n->d_bias[l]); }
-  fr(n->d_x8); fr(n->d_y); fr(n->d_g8); fr(n->d_ga); fr(n->d_gb); fr(n->d_part); fr(n->d_tmp3); fr(n->d_batch_t);
+  fr(n->d_x8); fr(n->d_y); fr(n->d_g8); fr(n->d_ga); fr(n->d_gb); fr(n->d_part); fr(n->d_wred); fr(n->d_tmp3);
+  fr(n->d_batch_t);
   for (int l = 0; l < kLayers - 1; ++l) fr(n->d_act[l]);
   fr(n->d_sx); fr(n->d_st); fr(n->d_sf);
   fr(n->d_dir); fr(n->d_cov); fr(n->d_fprev); fr(n->d_dprev);
