@@ -22,6 +22,12 @@ forward (orc_forward_*)    linearity in the source amplitude; mirror symmetry of
 adjoint (in gradient)      dot-product test <A x, y> = <x, A^T y> of the interior
                            operators (with and without sponge); fixed point f = fo
                            -> zero residual -> zero gradient; data_dynamic KATs
+point receivers (record    adjoint-state dot product <traces, rr> = sum_k s(k) z(k+1)
+and additive injection)    at the source cell (orc_adjoint_probe_f64), sponge on
+Ricker pulse               peak 1 at t0, zeros at t0 +- 1/(sqrt(2) pi fp), zero mean,
+                           spectrum peak at fp
+sponge                     gamma = 0 gives the bare leapfrog bit for bit; reflected
+                           energy against a wide-grid run far below the bare edge's
 laplacian (in gradient)    exact 4/h^2 on u = i^2 + j^2 interior cells (via a
                            one-level probe); KATs
 checkpointing              bitwise equality with the single-segment path
@@ -87,6 +93,7 @@ def lib():
         L.orc_apply_forward_op_f64.argtypes = [vp, _f64p, _f64p, _f64p]
         L.orc_apply_adjoint_op_f64.argtypes = [vp, _f64p, _f64p, _f64p]
         L.orc_laplacian_f64.argtypes = [vp, _f64p, _f64p]
+        L.orc_adjoint_probe_f64.argtypes = [vp, _f64p, ctypes.c_int, _i32p, _i32p, _f64p, ctypes.c_int, _f64p]
         L.orc_schedule.argtypes = [ctypes.c_int, ctypes.c_int, _f64p]
         L.orc_schedule.restype = ctypes.c_double
         for suf, p in (("f64", _f64p), ("f32", _f32p)):
@@ -270,6 +277,15 @@ def laplacian(cfg, u):
     L = np.zeros(cfg.nx * cfg.ny)
     lib().orc_laplacian_f64(ctypes.byref(cfg), np.ascontiguousarray(u, np.float64), L)
     return L
+
+
+def adjoint_probe(cfg, f, rcell, ranchor, rr, cell):
+    """The adjoint sweep alone, driven by residual traces rr[nt][nr]: z(k) at `cell` for every level."""
+    out = np.zeros(cfg.nt)
+    lib().orc_adjoint_probe_f64(ctypes.byref(cfg), np.ascontiguousarray(f, np.float64), len(rcell),
+                                np.ascontiguousarray(rcell, np.int32), np.ascontiguousarray(ranchor, np.int32),
+                                np.ascontiguousarray(rr, np.float64), int(cell), out)
+    return out
 
 
 def schedule(kind, k, params):

@@ -25,6 +25,8 @@
 // wrong (tests/test_oracle.py builds variants with -DORC_MUTATION=n):
 //  1 correlation pairs z(k+1) with Lu(k)      2 additive receiver injection
 //  3 source window one level longer           4 adjoint ignores f at neighbours
+//  5 Ricker with 2 pi fp in place of pi fp     6 sponge profile (d / W)^2, inverted
+//  7 residual of level k+1 injected into z(k)
 #ifndef ORC_MUTATION
 #define ORC_MUTATION 0
 #endif
@@ -99,7 +101,11 @@ struct Model {
 // d < W from the nearest edge.  No reference counterpart.
 inline double sponge_g(const tomo_config &c, int d) {
   if (c.sponge_width <= 0 || d >= c.sponge_width) return 0.0;
+#if ORC_MUTATION == 6
+  double x = double(d) / double(c.sponge_width);
+#else
   double x = double(c.sponge_width - d) / double(c.sponge_width);
+#endif
   return c.sponge_gamma * x * x;
 }
 
@@ -114,7 +120,7 @@ T pulse_at(const tomo_config &c, int k) {
   }
   // Ricker
   T t = T(k) * T(c.dt) - T(c.ricker_t0);
-  T pf = T(M_PI) * T(c.ricker_fp) * t;
+  T pf = T(ORC_MUTATION == 5 ? 2 * M_PI : M_PI) * T(c.ricker_fp) * t;
   T x = pf * pf;
   return (T(1) - T(2) * x) * std::exp(-x);
 }
@@ -263,6 +269,10 @@ void adjoint_step(const Model<T> &m, const T *z1, const T *z2, T *z0, int nr, co
   z0[nx - 1 + nx * (ny - 1)] = (z0[nx - 2 + nx * (ny - 1)] + z0[nx - 1 + nx * (ny - 2)]) / T(2);
 }
 
+// Residual level injected into z(k): level k itself (backpropagation2 reads
+// the residual of the level it writes, sirt-mimo.cu:724-740).
+inline int injected_level(int k) { return ORC_MUTATION == 7 ? k + 1 : k; }
+
 // ---------------------------------------------------------- one shot, all
 // Forward of one shot: traces[nt][nr] (initial_signal, sirt-mimo.cu:617-649),
 // optional history hist[nt][n2] with corners averaged.
@@ -340,7 +350,8 @@ void shot_gradient(const tomo_config &c, const Model<T> &m, const int *bbox, int
       if (k < 1 || k > nt - 2) continue;
       T *u = seg.data() + (size_t)q * n2;
       corners_mean(u, m.nx, m.ny);
-      adjoint_step(m, zl(k + 1), zl(k + 2), zl(k), nr, rcell, ranchor, rr.data() + (size_t)k * nr, tmp);
+      adjoint_step(m, zl(k + 1), zl(k + 2), zl(k), nr, rcell, ranchor,
+                   rr.data() + (size_t)injected_level(k) * nr, tmp);
       laplacian(u, L.data(), m.nx, m.ny, m.h);
       const T *z = zl(ORC_MUTATION == 1 ? k + 1 : k);
       for (int id = 0; id < n2; ++id) G[id] += z[id] * L[id];
@@ -665,6 +676,23 @@ extern "C" void orc_apply_adjoint_op_f64(const tomo_config *c, const double *f, 
   std::vector<double> zero(m.n2, 0.0), tmp;
   std::fill(x, x + m.n2, 0.0);
   adjoint_step<double>(m, y, zero.data(), x, 0, nullptr, nullptr, nullptr, tmp);
+}
+// The adjoint sweep of shot_gradient alone (levels nt-2 .. 1), driven by
+// given residual traces rr[nt][nr]: out[k] = z(k) at `cell` (0 for levels
+// the sweep does not form).  Probe for the adjoint-state dot-product test
+// <traces, rr> = sum_k s(k) z(k+1)(source cell).
+extern "C" void orc_adjoint_probe_f64(const tomo_config *c, const double *f, int nr, const int *rcell,
+                                      const int *ranchor, const double *rr, int cell, double *out) {
+  Model<double> m;
+  build_model(*c, f, m);
+  const int n2 = m.n2, nt = m.nt;
+  std::vector<double> zr(3 * (size_t)n2, 0.0), tmp;
+  auto zl = [&](int k) -> double * { return zr.data() + (size_t)(k % 3) * n2; };
+  std::fill(out, out + nt, 0.0);
+  for (int k = nt - 2; k >= 1; --k) {
+    adjoint_step(m, zl(k + 1), zl(k + 2), zl(k), nr, rcell, ranchor, rr + (size_t)injected_level(k) * nr, tmp);
+    out[k] = zl(k)[cell];
+  }
 }
 extern "C" void orc_laplacian_f64(const tomo_config *c, const double *u, double *L) {
   laplacian(u, L, c->nx, c->ny, double(c->h));

@@ -100,6 +100,116 @@ def test_pulse_reference_window():
     assert abs(p[k] - want) <= 1e-6 * abs(want)
 
 
+# ------------------------------------------- BASELINE-mode pieces (unit preset)
+# The unit preset's Ricker pulse, 20-cell sponge and point receivers have no
+# reference counterpart; these checks pin them on what the mathematics fixes.
+
+def unit_cfg(n, nt):
+    return orc.config_unit(tomo.TomoConfig(), n, nt)
+
+
+def test_ricker_closed_forms():
+    """r(t) = (1 - 2 pi^2 fp^2 t^2) exp(-pi^2 fp^2 t^2): r(t0) = 1, zeros at
+    t0 +- 1/(sqrt(2) pi fp), zero mean, amplitude spectrum peaking at fp."""
+    c = unit_cfg(64, 121)
+    c.dt, c.ricker_t0 = 0.5, 30.0
+    c.ricker_fp = 1.0 / (np.sqrt(2.0) * np.pi * 5.0)   # zeros 5 time units = 10 levels from t0
+    p = orc.pulse(c)
+    assert p[60] == 1.0 and p.argmax() == 60
+    assert abs(p[50]) < 1e-12 and abs(p[70]) < 1e-12, (p[50], p[70])
+    assert p[49] < 0 < p[51] and p[71] < 0 < p[69]
+    # zero mean and spectral peak on a fine, untruncated window
+    c.nt, c.dt, c.ricker_t0, c.ricker_fp = 4000, 0.05, 50.0, 0.1
+    p = orc.pulse(c)
+    assert abs(p.sum()) < 1e-12 * np.abs(p).sum(), p.sum()
+    nfft = 1 << 16
+    spec = np.abs(np.fft.rfft(p, nfft))
+    fpk = np.fft.rfftfreq(nfft, c.dt)[spec.argmax()]
+    assert abs(fpk - 0.1) <= 1.0 / (nfft * c.dt), fpk
+
+
+def test_sponge_zero_gamma_is_bare_leapfrog():
+    """With gamma = 0 the damped leapfrog a (w sum + (2 - 4w) u) - ab u(k-1)
+    has a = ab = 1: the traces of the bare rule, bit for bit, and the same
+    gradient (the adjoint's general form, to rounding)."""
+    n, nt = 64, 160
+    rng = np.random.default_rng(11)
+    f = np.zeros((n, n))
+    f[16:48, 16:48] = rng.uniform(-0.1, 0.1, (32, 32))
+    f = f.ravel()
+    cells = np.array([20 + n * 20, 40 + n * 22, 30 + n * 44], np.int32)
+    bbox = np.array([32, 32, 30, 30], np.int32)
+    c0 = unit_cfg(n, nt)
+    c0.sponge_width = 0
+    c1 = unit_cfg(n, nt)
+    c1.sponge_gamma = 0.0
+    t0 = orc.forward(c0, f, bbox, cells)
+    t1 = orc.forward(c1, f, bbox, cells)
+    np.testing.assert_array_equal(t0, t1)
+    meas = orc.forward(c0, np.zeros(n * n), bbox, cells)
+    G0, m0 = orc.gradient(c0, f, bbox, cells, cells, meas)
+    G1, m1 = orc.gradient(c1, f, bbox, cells, cells, meas)
+    np.testing.assert_array_equal(m0, m1)
+    assert np.abs(G0 - G1).max() <= 1e-12 * np.abs(G0).max()
+
+
+def _centre_trace(n, nt, width, gamma, offset=10):
+    c = unit_cfg(n, nt)
+    c.sponge_width, c.sponge_gamma = width, gamma
+    m = n // 2
+    bbox = np.array([m, m, m, m], np.int32)
+    cell = np.array([m + offset + n * m], np.int32)
+    return orc.forward(c, np.zeros(n * n), bbox, cell)[0][:, 0]
+
+
+def test_sponge_absorbs_outgoing_waves():
+    """A pulse from the centre of a 161^2 grid, recorded 10 cells away for
+    700 levels (long enough for the echoes of all four edges to come back).
+    The reflected part is the difference from the same run on a 761^2 grid,
+    whose edges are too far away to echo within 700 levels.  Measured: the
+    20-cell sponge (gamma 0.12) returns 3.9e-4 of the direct energy, the
+    bare one-way edge rule 2.1e-3; an inverted profile (damping rising
+    towards the interior, ending in a step) returns 3.6e-2."""
+    nt = 700
+    free = _centre_trace(161 + 600, nt, 0, 0.0)
+    bare = _centre_trace(161, nt, 0, 0.0)
+    sponge = _centre_trace(161, nt, 20, 0.12)
+    e_direct = (free ** 2).sum()
+    e_bare = ((bare - free) ** 2).sum() / e_direct
+    e_sponge = ((sponge - free) ** 2).sum() / e_direct
+    assert e_sponge < 1e-3, e_sponge
+    assert e_sponge < 0.3 * e_bare, (e_sponge, e_bare)
+
+
+def test_point_receiver_adjoint_state_dot_product():
+    """Forward u(k+1) = A u(k) - B u(k-1) + s(k) e_src, traces d(k) = u(k) at
+    the receivers; the adjoint sweep z(k) = A^T z(k+1) - B^T z(k+2) + P^T
+    rr(k) (additive injection at the receiver cells, gain 1) satisfies
+    <d, rr> = sum_k s(k) z(k+1)(src), with s(k) = v(src)^2 dt^2 pulse(k).
+    Variable field, sponge on; 90 levels, so the waves never reach the edge
+    rule (where the Neumann adjoint is not the forward's transpose) but run
+    8 cells into the sponge.  Pins the record levels, the injection's cell,
+    level, sign and gain, and the adjoint stepping with damping together."""
+    n, nt = 96, 90
+    c = unit_cfg(n, nt)
+    rng = np.random.default_rng(1)
+    f = np.zeros((n, n))
+    f[30:66, 30:66] = rng.uniform(-0.1, 0.15, (36, 36))
+    f = f.ravel()
+    src = 48 + n * 48
+    ang = np.linspace(0, 2 * np.pi, 12, endpoint=False)
+    cells = (np.round(48 + 12 * np.cos(ang)) + n * np.round(48 + 12 * np.sin(ang))).astype(np.int32)
+    tr = orc.forward(c, f, np.array([48, 48, 48, 48], np.int32), cells)[0]
+    rr = rng.standard_normal((nt, len(cells)))
+    rr[:c.record_first] = 0.0
+    rr[nt - 1] = 0.0   # the sweep starts at level nt - 2
+    z = orc.adjoint_probe(c, f, cells, cells, rr, src)
+    s = (1.0 + f[src]) * c.c0 ** 2 * c.dt ** 2 * orc.pulse(c)
+    lhs = float((tr * rr).sum())
+    rhs = float(sum(s[k] * z[k + 1] for k in range(1, nt - 1)))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
 # ------------------------------------------------------------------ forward
 
 def test_forward_superposition():
@@ -364,11 +474,15 @@ _MUTANT_SCRIPT = r"""
 import sys
 sys.path[:0] = [{tests!r}, {oracle!r}]
 import test_oracle as t
+checks = {{
+    "kat": lambda: t.test_kat_iteration1(10),
+    "dot": lambda: t.test_adjoint_dot_product("reference"),
+    "ricker": t.test_ricker_closed_forms,
+    "sponge": t.test_sponge_absorbs_outgoing_waves,
+    "inject": t.test_point_receiver_adjoint_state_dot_product,
+}}
 try:
-    if sys.argv[1] == "kat":
-        t.test_kat_iteration1(10)
-    else:
-        t.test_adjoint_dot_product("reference")
+    checks[sys.argv[1]]()
     print("passed")
 except AssertionError:
     print("failed")
@@ -376,7 +490,8 @@ except AssertionError:
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("mutation,check", [(1, "kat"), (2, "kat"), (3, "kat"), (4, "dot")])
+@pytest.mark.parametrize("mutation,check", [(1, "kat"), (2, "kat"), (3, "kat"), (4, "dot"), (5, "ricker"),
+                                            (6, "sponge"), (7, "inject"), (7, "kat")])
 def test_planted_mistakes_are_caught(mutation, check):
     """The same KAT / dot-product test must fail on an oracle with a planted mistake."""
     src = [os.path.join(orc.HERE, f) for f in ("tomo_oracle.cpp", "cnn_oracle.cpp")]
