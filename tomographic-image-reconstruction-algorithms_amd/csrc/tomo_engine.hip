@@ -222,9 +222,11 @@ int upload_receivers(tomo_ctx *c) {
   HIPCHK(c, hipMemcpyAsync(c->d_rowflag, rowflag.data(), c->ny, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->recvseg_ok = false;
+  // host source of an asynchronous copy: lives until the synchronisation below
+  std::vector<int> seg;
   if (adjoint2_supported(c->nx, c->cfg.sponge_width)) {
     const int vec = c->nx / 64;
-    std::vector<int> seg((size_t)c->ny * 64, -1);
+    seg.assign((size_t)c->ny * 64, -1);
     bool ok = true;
     for (size_t r = 0; r < c->rcell.size(); ++r) {
       const int x = c->rcell[r] % c->nx, y = c->rcell[r] / c->nx;
