@@ -272,6 +272,29 @@ int tomo_update_uniform(tomo_ctx *ctx, float scale, int mode);
  * weights_out (may be NULL) receives the normalised weights, double[Ng]. */
 int tomo_update_weighted(tomo_ctx *ctx, float scale, double *weights_out);
 
+/* The update of tomo_update_weighted on caller-owned DEVICE arrays, with no
+ * device allocation: no counterpart in the reference; for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).
+ * d_f float[ny][nx] is updated in place from d_grad float[nshots][ny][nx]
+ * (raw per-shot gradients G_m); mask4 (host) = {x0, x1, y0, y1} half-open;
+ * d_pixw (device, may be NULL) multiplies each pixel's step, as the
+ * coverage weights of tomo_set_pixel_weights do; d_weights (device, may be
+ * NULL) receives the normalised weights, double[nshots].  Same kernels,
+ * order and bits as tomo_update_weighted.  The FP64 scratch (mean image,
+ * partial distances, weights) lives in one static device array of
+ * TOMO_NOALLOC_SCRATCH_DOUBLES per GPU, held until the process ends: calls
+ * needing nx*ny + nshots*(p + 1) + 1 doubles beyond it (p = partial blocks,
+ * 64 at 512 x 512; e.g. 512 x 512 with up to 4032 shots fits) return
+ * TOMO_ERR_ARG, with the message in tomo_last_error(NULL).  Work is queued
+ * on `stream` (a hipStream_t; NULL = the null stream) and not waited for.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams or host threads) race.  The first call on a device
+ * looks up the array's address (hipGetSymbolAddress); make it once before
+ * any graph capture. */
+#define TOMO_NOALLOC_SCRATCH_DOUBLES (1 << 19) /* 4 MiB */
+int tomo_update_weighted_noalloc(float *d_f, const float *d_grad, int nshots, int nx, int ny, const int *mask4,
+                                 double scale, const float *d_pixw, double *d_weights, void *stream);
+
 /* Weighted average of intermediate fields (sirt-mimo-avg-weight.cu:80-203,
  * 530-544): fg_g = f + scale * mask * G_g / (1 + f) per group;
  * w_g = || fg_g - mean_g fg_g ||_2 over the full grid; f += sum_g fg_g / w_g

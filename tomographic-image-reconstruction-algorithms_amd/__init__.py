@@ -113,6 +113,8 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "tomo_update_uniform": (_int, [_vp, ctypes.c_float, _int]),
     "tomo_update_weighted": (_int, [_vp, ctypes.c_float, _vp]),
+    # device pointers (d_f, d_grad, d_pixw, d_weights) and the stream as plain addresses
+    "tomo_update_weighted_noalloc": (_int, [_vp, _vp, _int, _int, _int, _i32p, ctypes.c_double, _vp, _vp, _vp]),
     "tomo_update_avg_weight": (_int, [_vp, ctypes.c_float, _vp]),
     "tomo_update_group": (_int, [_vp, _int, ctypes.c_float]),
     "tomo_error": (_int, [_vp, ctypes.POINTER(ctypes.c_double)]),

@@ -33,7 +33,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]
 
-HIP_SOURCES = ["tomo_kernels.hip", "tomo_engine.hip", "tomo_cnn.hip", "tomo_batch.hip"]
+HIP_SOURCES = ["tomo_kernels.hip", "tomo_engine.hip", "tomo_cnn.hip", "tomo_batch.hip", "tomo_noalloc.hip"]
 HOST_SOURCES = ["tomo_host.cpp"]
 LIB = os.path.join(OUT, "libtomo_hip.so")
 
