@@ -13,18 +13,26 @@ checkpointed every 200 steps, smooth random velocity (Gaussian-filtered noise,
 sigma 8 cells, +-8 %) plus 4 sharp inclusions; measured traces from the FP64
 solver.
 
-Also reported: the roofline of the dominant stencil kernel (the bytes per
-cell-update the algorithm must move under its blocking, the average launch
-time from HIP events on the engine's stream, and the measured HBM bytes per
-launch from the committed counter summary), both stencil kernels' figures,
-and the CPU baseline (the oracle's FP32 transcription, OpenMP over shots) on
-a bounded sample of the same workload.
+Also reported: the roofline of the dominant stencil kernel (the average
+launch time from HIP events on the engine's stream, with three labelled
+fractions of the HBM peak: at the per-unit bytes of the one-step
+formulation, at the bytes the blocked kernels must move, and at the HBM bytes
+per launch measured by the committed counter summary), both stencil kernels'
+figures, and the CPU baseline (the oracle's FP32 transcription built for the
+host CPU, OpenMP over shots) on a bounded sample of the same workload.
+
+No process that runs the engine imports torch: torch ships its own HIP and
+HSA runtimes under the same sonames as ROCm's, and the engine would bind to
+those instead of the ROCm runtime it is built against and the profiler
+instruments.  The calibration copy uses the engine's own HIP runtime.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -39,8 +47,8 @@ from _pkg import tomo  # noqa: E402
 import tomo_amd.workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell-update of one shot that the stencil algorithm must move
-# (roofline.achieved): the coefficient image is shared by every shot of a
+# HBM bytes per cell-update of one shot that the blocked stencil kernels must
+# move (fractions.blocked_algorithmic): the coefficient image is shared by every shot of a
 # launch (4/shots B, L2-resident); the two-step forward kernel reads u(k),
 # u(k-1) and writes u(k+1), u(k+2) once per two steps (8 B), the one-step
 # kernel 12 B; with three steps a recording pass writes only u(k+2), u(k+3)
@@ -51,12 +59,12 @@ HBM_PEAK_GBS = 8000.0
 # re-writes the gradient each step (24 B).
 FWD_BYTES = {1: 12.0, 2: 8.0, 3: 6.0}
 ADJ_BYTES = {1: 24.0, 2: 16.0}
-# The per-unit figures of the one-step formulation, reported beside them: a
+# The per-unit figures of the one-step formulation (roofline.achieved): a
 # forward cell-update reads u(k), u(k-1) and the per-cell coefficient and
 # writes u(k+1) (16 B); an adjoint cell-update fused with the correlation
 # reads z(k+1), z(k+2), the coefficient and u(k) and writes z(k), with the
-# gradient in registers (20 B).  With several steps per launch these exceed what
-# the kernels must move.
+# gradient in registers (20 B).  With several steps per launch these exceed
+# what the kernels must move (the forward's then exceeds the HBM peak).
 FWD_UNIT_BYTES = 16.0
 ADJ_UNIT_BYTES = 20.0
 # Measured HBM bytes per launch of the stencil kernels (rocprofv3 --pmc
@@ -72,6 +80,42 @@ WORKLOADS = {
 }
 
 
+def hip_runtime():
+    """The HIP runtime the engine library is linked against (the process's
+    libamdhip64.so.7, loaded with the library)."""
+    tomo.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    return hip
+
+
+def device_copy(nbytes):
+    """One device-to-device copy of nbytes (a plain copy of known size: its
+    counter readings scale FETCH_SIZE / WRITE_SIZE)."""
+    hip = hip_runtime()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipDeviceSynchronize() or hip.hipMemcpy(b, a, nbytes, 3) or hip.hipDeviceSynchronize():
+            raise RuntimeError("device copy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def native_oracle():
+    """The oracle built for this host's CPU (-march=native, FMA contraction
+    allowed): the timed CPU baseline.  Built here, on the machine that runs
+    the bench, since the CPU that runs it is not known in advance."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("native oracle build failed:\n" + r.stdout + r.stderr)
+    return os.path.join(ROOT, "oracle", "_build", "liboracle_native.so")
+
+
 def make_workload(name):
     c, ii, jj, fo = W.problem(name)
     return c, ii, jj, fo
@@ -79,7 +123,8 @@ def make_workload(name):
 
 def cpu_baseline(c, ii, jj, fo, meas, threads, shots):
     """Oracle FP32 gradient of `shots` shots of the workload (same geometry,
-    steps and checkpointing), OpenMP over shots."""
+    steps and checkpointing), OpenMP over shots, on the oracle built for this
+    host (native_oracle)."""
     import oracle as orc
     orc.set_threads(threads)
     bbox = np.stack([ii, ii, jj, jj], axis=1).astype(np.int32).ravel()[: 4 * shots]
@@ -114,16 +159,53 @@ def load_traffic(kernel, shape, tb):
     return None
 
 
+def kernel_line(kernel, unit_bytes, blocked_bytes, cells_launch, avg_ms, total_ms, traffic, traffic_of):
+    """Roofline figures of one stencil kernel: `achieved` at the per-unit
+    bytes of the one-step formulation (forward 16 B, adjoint with its
+    correlation 20 B per cell-update) times the cell-updates of one launch,
+    over the average launch time; beside it the same launch time at the
+    bytes the blocked kernels must move, and at the HBM bytes per launch the
+    counters measured (traffic; None without a counter row for this shape)."""
+    sec = avg_ms * 1e-3
+    unit = unit_bytes * cells_launch / sec / 1e9
+    blocked = blocked_bytes * cells_launch / sec / 1e9
+    counted = traffic / sec / 1e9 if traffic else None
+    return {
+        "kernel": kernel,
+        "achieved": round(unit, 1), "frac": round(unit / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "traffic_of": traffic_of,
+        "fractions": {"per_unit": round(unit / HBM_PEAK_GBS, 4),
+                      "blocked_algorithmic": round(blocked / HBM_PEAK_GBS, 4),
+                      "counter_measured": round(counted / HBM_PEAK_GBS, 4) if counted else None},
+        "bytes_per_cell_update": {"per_unit": unit_bytes, "blocked_algorithmic": blocked_bytes,
+                                  "counter_measured": round(traffic / cells_launch, 3) if traffic else None},
+        "cells_per_launch": cells_launch, "avg_launch_ms": round(avg_ms, 5), "total_ms": round(total_ms, 2),
+    }
+
+
 def cpu_model():
-    """The host CPU's model name (Linux), or None."""
+    """The host CPU as /proc/cpuinfo names it (model name, family / model /
+    stepping, sockets), with the logical CPU count and this process's
+    affinity, or None."""
+    info = {}
+    sockets = set()
     try:
         with open("/proc/cpuinfo") as fh:
             for ln in fh:
-                if ln.startswith("model name"):
-                    return ln.split(":", 1)[1].strip()
+                if ":" not in ln:
+                    continue
+                k, v = (x.strip() for x in ln.split(":", 1))
+                if k == "physical id":
+                    sockets.add(v)
+                elif k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in info:
+                    info[k] = v
     except OSError:
-        pass
-    return None
+        return None
+    if not info:
+        return None
+    return (f"{info.get('model name', '?')} (vendor {info.get('vendor_id', '?')}, family {info.get('cpu family', '?')}, "
+            f"model {info.get('model', '?')}, stepping {info.get('stepping', '?')}; {max(len(sockets), 1)} socket(s), "
+            f"{os.cpu_count()} logical CPUs, {len(os.sched_getaffinity(0))} in this process's affinity)")
 
 
 def counter_pass(args):
@@ -132,7 +214,6 @@ def counter_pass(args):
     copy: the program of the rocprofv3 --pmc runs that measure the stencil
     kernels' HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE; the copy's
     reading gives the counters' scale)."""
-    import torch
     log = (lambda m: print("[counters]", m, file=sys.stderr, flush=True))
     c, ii, jj, fo = make_workload("c3")
     e = tomo.Engine(c, device=0)
@@ -157,12 +238,9 @@ def counter_pass(args):
     tb, ab = e.time_blocking(), e.adjoint_blocking()
     e.close()
     log("copy 1 GiB")
-    a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
+    device_copy(1 << 30)
     print(json.dumps({"counter_pass": "c3", "stencil_shape": shape, "forward_steps_per_launch": tb,
-                      "adjoint_steps_per_launch": ab, "copy_bytes": 2 * a.numel() * 4}), flush=True)
+                      "adjoint_steps_per_launch": ab, "copy_bytes": 2 << 30}), flush=True)
 
 
 def main():
@@ -288,10 +366,6 @@ def main():
     adj_cells_launch = shots * n * n * ab
     fwd_avg_ms = tm["fwd_ms"] / max(tm["fwd_launches"], 1) * tb
     adj_avg_ms = tm["adj_ms"] / max(tm["adj_launches"], 1) * ab
-    fwd_gbs = FWD_BYTES[tb] * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
-    adj_gbs = ADJ_BYTES[ab] * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
-    fwd_unit_gbs = FWD_UNIT_BYTES * fwd_cells_launch / (fwd_avg_ms * 1e-3) / 1e9
-    adj_unit_gbs = ADJ_UNIT_BYTES * adj_cells_launch / (adj_avg_ms * 1e-3) / 1e9
     fwd_kernel = {1: "forward_vec_kernel", 2: "forward2_kernel", 3: "forward3_kernel"}[tb]
     adj_kernel = "adjoint2_kernel" if ab == 2 else "adjoint_vec_kernel"
     fwd_traffic = load_traffic(fwd_kernel, shape, tb) if args.workload == "c3" else None
@@ -324,21 +398,12 @@ def main():
                                                             / 1e9, 3)},
         },
         "roofline": None,
-        "forward": {
-            "kernel": f"{fwd_kernel} (with its edge/receiver passes)",
-            "achieved": round(fwd_gbs, 1), "frac": round(fwd_gbs / HBM_PEAK_GBS, 4), "traffic": fwd_traffic,
-            "traffic_of": "recording launches" if tb == 3 else "every launch",
-            "bytes_per_cell_update": FWD_BYTES[tb], "cells_per_launch": fwd_cells_launch,
-            "avg_launch_ms": round(fwd_avg_ms, 5), "total_ms": round(tm["fwd_ms"], 2),
-            "unit_bytes_per_cell_update": FWD_UNIT_BYTES, "unit_achieved": round(fwd_unit_gbs, 1),
-        },
-        "adjoint": {
-            "kernel": f"{adj_kernel} (with its edge/receiver pass)",
-            "achieved": round(adj_gbs, 1), "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "traffic": adj_traffic,
-            "bytes_per_cell_update": ADJ_BYTES[ab], "cells_per_launch": adj_cells_launch,
-            "avg_launch_ms": round(adj_avg_ms, 5), "total_ms": round(tm["adj_ms"], 2),
-            "unit_bytes_per_cell_update": ADJ_UNIT_BYTES, "unit_achieved": round(adj_unit_gbs, 1),
-        },
+        "forward": kernel_line(f"{fwd_kernel} (with its edge/receiver passes)", FWD_UNIT_BYTES, FWD_BYTES[tb],
+                               fwd_cells_launch, fwd_avg_ms, tm["fwd_ms"], fwd_traffic,
+                               "recording and replay launches as one gradient mixes them" if tb == 3
+                               else "every launch"),
+        "adjoint": kernel_line(f"{adj_kernel} (with its edge/receiver pass)", ADJ_UNIT_BYTES, ADJ_BYTES[ab],
+                               adj_cells_launch, adj_avg_ms, tm["adj_ms"], adj_traffic, "every launch"),
         "cpu_baseline": None,
     }
 
@@ -348,13 +413,17 @@ def main():
     out["roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", **out[dom]}
     if not args.no_cpu_baseline:
         import oracle as orc
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        sample = threads
+        orc.use_library(native_oracle())
+        # every OpenMP thread of this process (OMP_NUM_THREADS, or the CPUs
+        # of its affinity), one shot each
+        threads = args.cpu_threads or orc.num_threads()
+        sample = min(threads, shots)
         t_cpu = cpu_baseline(c, ii, jj, fo, meas, threads, sample)
         cpu_cells = cells / args.steps / shots * sample
         out["cpu_baseline"] = {
             "value": round(cpu_cells / t_cpu / 1e9, 4), "unit": "Gcell-updates/s", "cores": orc.num_threads(),
             "kind": "port",
+            "build": "oracle/tomo_oracle.cpp, g++ -O3 -march=native -ffp-contract=fast -fopenmp, built on this host",
             "sample": f"oracle FP32 gradient (forward + checkpoint recompute + adjoint) of {sample} of the "
                       f"{shots} shots, {n}x{n}, {steps} steps: {t_cpu:.2f} s (the whole iteration, "
                       f"{shots / sample:g}x the shots, would take {t_cpu * shots / sample:.0f} s at this rate)",
@@ -362,13 +431,8 @@ def main():
         }
     e.close()
     if args.copy_calibration:
-        import torch
-        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
         print("[bench] copy calibration: 1 GiB device copy", file=sys.stderr, flush=True)
-        b.copy_(a)
-        torch.cuda.synchronize()
+        device_copy(1 << 30)
     print(json.dumps(out), flush=True)
 
 

@@ -422,7 +422,11 @@ int tomo_get_shape(const tomo_ctx *ctx, int *out5);
  * and measured traces set, the sponge preset and the vectorised kernels
  * (nx a multiple of 2 or 4, nx >= 8), one checkpoint interval for all.  The
  * launch program is built at creation for the contexts' buffers and shapes
- * as they are then (set the shapes, e.g. with tomo_autotune, before). */
+ * as they are then (set the shapes, e.g. with tomo_autotune, before);
+ * tomo_batch_gradient returns TOMO_ERR_STATE once any context has changed
+ * them (shots, receivers, coverage, shape ...).  Programs of more than
+ * 16384 launches per gradient call are refused at creation (TOMO_ERR_ARG):
+ * split such sets into several batches. */
 typedef struct tomo_batch tomo_batch;
 
 int tomo_batch_create(tomo_ctx *const *ctxs, int n, tomo_batch **out);

@@ -66,6 +66,14 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
+def use_library(path: str) -> None:
+    """Load the oracle from `path` (e.g. the host-native build of the timed
+    CPU baseline) instead of the checker build; call before the first use."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (g++ + OpenMP)."""
     if force or not os.path.exists(LIB_PATH):

@@ -73,3 +73,44 @@ def test_batch_rejects_mixed_checkpoints():
     finally:
         a.close()
         e.close()
+
+
+def test_batch_refuses_changed_contexts():
+    """The program holds the contexts' device pointers and shapes as they were
+    at creation: after a context changes them, the batch must refuse to run."""
+    print("[case] batch after a context changed", flush=True)
+    a = problem(64, 16, 512, 3, False)
+    b = problem(96, 24, 768, 4, False)
+    try:
+        with tomo.Batch([a, b]) as bt:
+            bt.gradient()
+            a.enable_coverage(True)
+            with pytest.raises(tomo.TomoError, match="changed"):
+                bt.gradient()
+        with tomo.Batch([a, b]) as bt:   # built for the new state: runs
+            bt.gradient()
+            b.set_groups(2)              # fewer shots: per-shot buffers and tables rebuilt
+            with pytest.raises(tomo.TomoError, match="changed"):
+                bt.gradient()
+    finally:
+        a.close()
+        b.close()
+
+
+def test_batch_refuses_oversized_program():
+    """A program above the accepted launch count is refused at creation,
+    before any launch (one context of 6000 steps: about 18k launches)."""
+    print("[case] batch program size limit", flush=True)
+    c = unit_config(64, 6002)
+    c.checkpoint_every = 100
+    ii, jj = tomo.transducers_ring(64, 64, 4, 25.0)
+    e = tomo.Engine(c, device=0)
+    try:
+        e.set_transducers(ii, jj, one_based=False)
+        e.set_receivers(ii, jj, one_based=False)
+        e.set_groups(1)
+        e.set_measured(np.zeros((4, 6002, 4), np.float32))
+        with pytest.raises(tomo.TomoError, match="launch program"):
+            tomo.Batch([e])
+    finally:
+        e.close()

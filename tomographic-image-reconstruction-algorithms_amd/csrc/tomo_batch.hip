@@ -42,9 +42,18 @@ AdjArgs adjoint_args(tomo_ctx *c, int shot0);
 float pulse_h2_f32(const tomo_ctx *c, int k);
 int wavefields(tomo_ctx *c, int P);
 int shots_per_pass(const tomo_ctx *c);
+std::vector<uintptr_t> context_signature(const tomo_ctx *c);
 }  // namespace tomo_internal
 
 namespace {
+
+// Largest launch program accepted (kernel launches plus fills per gradient
+// call).  The largest programs run so far are the engine's captured gradient
+// passes of about 3 nt launches (12k at nt = 4000); the one much larger batch
+// program attempted (about 260k launches in one graph, 64 contexts of up to
+// 512 x 512 and 4096 steps) ended in a host-side crash whose cause is not
+// established, so programs beyond this size are refused.
+constexpr double kMaxProgramLaunches = 16384;
 
 struct Rec {
   int kind;            // 0 forward step, 1 adjoint step
@@ -75,6 +84,7 @@ struct tomo_batch {
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<tomo_ctx *> ctx;
+  std::vector<std::vector<uintptr_t>> sig;  // context_signature of each context at build
   std::vector<int> P;
   std::vector<Pass> passes;
   std::vector<Rec> recs;
@@ -258,6 +268,15 @@ int build(tomo_batch *b) {
     pass.seg_end = b->recs.size();
     b->passes.push_back(pass);
   }
+  // size check before anything is allocated for the program
+  double launches = (double)b->recs.size();
+  for (const auto &pass : b->passes)
+    for (const auto &it : pass.items) launches += 8 + (b->ctx[it.ctx]->coverage ? 1 : 0);
+  if (launches > kMaxProgramLaunches)
+    return bfail(b, TOMO_ERR_ARG,
+                 "launch program of " + std::to_string((long long)launches) + " launches per gradient is above the " +
+                     std::to_string((long long)kMaxProgramLaunches) + " accepted; split the contexts into smaller batches");
+  for (tomo_ctx *c : b->ctx) b->sig.push_back(context_signature(c));
   // device tables
   BCHK(b, hipMalloc((void **)&b->d_fwd, sizeof(FwdArgs<float>) * std::max<size_t>(b->hfwd.size(), 1)));
   BCHK(b, hipMalloc((void **)&b->d_adj, sizeof(AdjArgs) * std::max<size_t>(b->hadj.size(), 1)));
@@ -382,6 +401,12 @@ int tomo_batch_set_graph(tomo_batch *b, int on) {
 
 int tomo_batch_gradient(tomo_batch *b, double *misfits) {
   if (!b) return TOMO_ERR_ARG;
+  // the program holds the contexts' device pointers and shapes as they were
+  // at creation: refuse to run once any of them changed
+  for (size_t i = 0; i < b->ctx.size(); ++i)
+    if (context_signature(b->ctx[i]) != b->sig[i])
+      return bfail(b, TOMO_ERR_STATE, "context " + std::to_string(i) +
+                                          " changed its buffers, geometry or shape since the batch was created");
   BCHK(b, hipSetDevice(b->device));
   for (tomo_ctx *c : b->ctx) BCHK(b, hipStreamSynchronize(c->stream));
   BCHK(b, hipEventRecord(b->ev0, b->stream));

@@ -879,12 +879,10 @@ int check_ctx(tomo_ctx *c) {
   return TOMO_OK;
 }
 
+// the buffers, geometry and shapes a launch program built for c depends on
+std::vector<uintptr_t> context_signature(const tomo_ctx *c) { return graph_signature(c); }
+
 }  // namespace tomo_internal
-
-namespace {
-
-
-}  // namespace
 
 namespace tomo_internal {
 
