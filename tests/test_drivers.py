@@ -164,3 +164,105 @@ def test_generate_data_then_sirt_and_art(tmp_path):
     assert r.returncode == 0, r.stderr
     row = [l.split() for l in r.stdout.splitlines()[2:] if l.strip()][0]
     assert row[0] == "1" and abs(float(row[1]) - 83.657) < 0.05 and abs(float(row[3]) - 22500) <= 300
+
+
+def _side_traces(e, path, ng, G=160):
+    sides = [tomo.read_text(path / f"fo-data-{s}-{G}.txt", (192, 401, ng)).reshape(ng, 401, 192)
+             for s in ("bottom", "right", "top", "left")]
+    return e.sides_to_traces(*sides)
+
+
+@pytest.mark.gpu
+def test_generate_data_fp64_and_noise_against_oracle(tmp_path):
+    """generate-data --fp64 at G = 160 writes the oracle's FP64 traces (to the
+    float rounding of the files); --noise <rel> <seed> adds to each receiver
+    trace seeded Gaussian noise of standard deviation rel * RMS(trace), the
+    same bits as tomo_add_noise on the clean traces (generate-data.cu:102-224;
+    the reference's noisy generators are unseeded)."""
+    import oracle as orc
+    print("[case] generate-data --fp64 / --noise G=160", flush=True)
+    clean_dir, noisy_dir, again_dir = tmp_path / "clean", tmp_path / "noisy", tmp_path / "again"
+    for d in (clean_dir, noisy_dir, again_dir):
+        d.mkdir()
+        assert run(["generate-field"], d).returncode == 0
+    r = run(["generate-data", "fo.txt", 160, "--fp64"], clean_dir)
+    assert r.returncode == 0, r.stderr
+    r = run(["generate-data", "fo.txt", 160, "--fp64", "--noise", 0.1, 7], noisy_dir)
+    assert r.returncode == 0, r.stderr
+    r = run(["generate-data", "fo.txt", 160, "--noise", 0.1, 8, "--fp64"], again_dir)
+    assert r.returncode == 0, r.stderr
+    r = run(["generate-data", "fo.txt", 160, "--noise", "x", 8], again_dir)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    c = orc.config_reference(tomo.TomoConfig())
+    ii, jj = orc.transducers_reference()
+    rc, _ = orc.receivers_lines(c)
+    bbox = orc.groups(ii, jj, 160)
+    fo = orc.phantom_reference()
+    want = orc.forward(c, fo, bbox, rc, np.float64)
+    with tomo.Engine(tomo.config_reference()) as e:
+        e.set_transducers(*tomo.transducers_reference(), one_based=True)
+        ng = e.set_groups(160)
+        clean = _side_traces(e, clean_dir, ng)
+        noisy = _side_traces(e, noisy_dir, ng)
+        other = _side_traces(e, again_dir, ng)
+    rel = float(np.linalg.norm(clean - want) / np.linalg.norm(want))
+    print(f"  clean FP64 files vs oracle: rel L2 {rel:.2e}", flush=True)
+    assert rel < 1e-7
+    np.testing.assert_array_equal(noisy, tomo.add_noise(clean.copy(), 0.1, 7))
+    assert not np.array_equal(noisy, other)
+    d = (noisy.astype(np.float64) - clean).transpose(0, 2, 1).reshape(-1, 401)
+    rms = np.sqrt((clean.astype(np.float64).transpose(0, 2, 1).reshape(-1, 401) ** 2).mean(1))
+    live = rms > 0
+    ratio = d[live].std(1) / rms[live]
+    print(f"  noise sd / trace RMS: mean {ratio.mean():.4f}, min {ratio.min():.3f}, max {ratio.max():.3f}", flush=True)
+    assert abs(ratio.mean() - 0.1) < 0.002 and 0.07 < ratio.min() and ratio.max() < 0.13
+
+
+@pytest.mark.gpu
+def test_sirt_mimo_seq_trajectory_matches_oracle(tmp_path):
+    """sirt-mimo-seq (sirt-mimo-seq.cu:337-596): 5 ART iterations (the 4 groups
+    of G = 160 in order, f += s G_g / (1 + f) after each) then non-uniform
+    weighted SIRT, s = omega (alpha + beta / (k + 1)^gamma) = 20000, 7
+    iterations and the reference stop rules; the same loop on the FP64 oracle
+    with the driver's own data files.  Per-iteration epsilon and the final
+    field file against the oracle."""
+    import oracle as orc
+    print("[case] sirt-mimo-seq trajectory G=160", flush=True)
+    assert run(["generate-field"], tmp_path).returncode == 0
+    assert run(["generate-data", "fo.txt", 160], tmp_path).returncode == 0
+    r = run(["sirt-mimo-seq", "fo.txt", 160, 0, 7, 1, 20000, 0, 1], tmp_path)
+    assert r.returncode == 0, r.stderr
+    eps_drv = table_eps(r.stdout)
+    f_drv = np.loadtxt(tmp_path / "sirt-norm-160-fo.txt", dtype=np.float32).ravel()
+    c = orc.config_reference(tomo.TomoConfig())
+    ii, jj = orc.transducers_reference()
+    rc, ra = orc.receivers_lines(c)
+    bbox = orc.groups(ii, jj, 160)
+    fo = tomo.read_text(tmp_path / "fo.txt", (192, 192))
+    with tomo.Engine(tomo.config_reference()) as e:
+        e.set_transducers(ii, jj, one_based=True)
+        meas = _side_traces(e, tmp_path, e.set_groups(160)).astype(np.float64)
+    f = np.zeros(fo.size)
+    eps_orc = []
+    prev = 100.0
+    for it in range(7):
+        if it < 5:
+            for g in range(4):
+                G, _ = orc.gradient(c, f, bbox[4 * g:4 * g + 4], rc, ra, meas[g:g + 1], np.float64)
+                orc.update_group(c, f, G[0], 20000.0)
+        else:
+            G, _ = orc.gradient(c, f, bbox, rc, ra, meas, np.float64)
+            orc.update_weighted(c, f, G, 20000.0)
+        eps = orc.error(f, fo)
+        eps_orc.append(eps)
+        if eps > prev:
+            break
+        prev = eps
+    print("  driver", [round(v, 4) for v in eps_drv], flush=True)
+    print("  oracle", [round(v, 4) for v in eps_orc], flush=True)
+    assert len(eps_drv) == len(eps_orc), (eps_drv, eps_orc)
+    np.testing.assert_allclose(eps_drv, eps_orc, rtol=0, atol=5e-3)
+    if len(eps_orc) == 7:
+        rel = float(np.linalg.norm(f_drv - f) / np.linalg.norm(f))
+        print(f"  final field rel L2 {rel:.2e}", flush=True)
+        assert rel < 5e-3
