@@ -58,7 +58,7 @@ HBM_PEAK_GBS = 8000.0
 # the gradient once per two steps (16 B), the one-step adjoint re-reads and
 # re-writes the gradient each step (24 B).
 FWD_BYTES = {1: 12.0, 2: 8.0, 3: 6.0}
-ADJ_BYTES = {1: 24.0, 2: 16.0}
+ADJ_BYTES = {1: 24.0, 2: 16.0, 3: 12.0}
 # The per-unit figures of the one-step formulation (roofline.achieved): a
 # forward cell-update reads u(k), u(k-1) and the per-cell coefficient and
 # writes u(k+1) (16 B); an adjoint cell-update fused with the correlation
@@ -258,7 +258,11 @@ def main():
     ap.add_argument("--out", default="preconditioner_trained.npy", help="parameters written by --study train")
     ap.add_argument("--phantoms", type=int, default=48, help="training phantoms of --study train")
     ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of --study train")
+    ap.add_argument("--train-target", default="best",
+                    help="--study train targets: best (the trajectory's best iterate) or ahead:K (K iterations on)")
     ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
+    ap.add_argument("--learned-iters", type=int, default=3,
+                    help="--study ttt: learned iterations (with the plain fallback) before plain ones")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--time-blocking", type=int, default=3, choices=[1, 2, 3],
                     help="forward steps per launch (2, 3: the intermediate levels in registers)")
@@ -290,18 +294,30 @@ def main():
         if args.study == "c2":
             out = S.c2_study(log=log)
         elif args.study == "train":
-            out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps, log=log)
+            out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
+                                target=args.train_target, log=log)
         elif args.study == "finetune":
             out = S.finetune_study(args.out, base=args.weights, phantoms=args.phantoms,
                                    train_steps=args.train_steps, log=log)
         else:
-            runs = [S.ttt_study(seed=int(v), weights=args.weights, log=log) for v in args.seeds.split(",")]
-            sp = [r["speedup"] for r in runs if r["speedup"] is not None]
-            out = {"study": "ttt", "runs": runs,
-                   "speedups": [r["speedup"] for r in runs],
-                   "speedups_fixed50": [r["speedup_fixed50"] for r in runs if r["speedup_fixed50"] is not None],
-                   "median_speedup": float(np.median(sp)) if sp else None,
-                   "all_reached": all(r["learned_seconds"] is not None for r in runs)}
+            runs = []
+            for v in args.seeds.split(","):
+                runs.append(S.ttt_study(seed=int(v), weights=args.weights, learned_iters=args.learned_iters, log=log))
+                # one line per seed as it finishes (the whole study runs for minutes)
+                print(json.dumps({"ttt_seed": int(v), **{k: runs[-1][k] for k in (
+                    "plain_never_below_100", "speedup_fixed50", "speedup_stop_rule", "speedup_fixed10")}}),
+                      file=sys.stderr, flush=True)
+            out = {"study": "ttt", "runs": runs, "flagged_never_below_100": [r["phantom_seed"] for r in runs
+                                                                             if r["plain_never_below_100"]]}
+            for name in ("fixed50", "stop_rule", "fixed10"):
+                counted = [r for r in runs if not r["plain_never_below_100"]]
+                sp = [r["targets"][name]["speedup"] for r in counted]
+                reached = [x for x in sp if x is not None]
+                # a seed whose learned run never reaches the target counts as a speedup of 0 in the median
+                out[name] = {"speedups": sp, "reached": f"{len(reached)} of {len(counted)}",
+                             "at_least_3x": sum(1 for x in reached if x >= 3.0),
+                             "median_speedup": float(np.median([x if x is not None else 0.0 for x in sp]))
+                             if sp else None}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "c4":
@@ -367,7 +383,7 @@ def main():
     fwd_avg_ms = tm["fwd_ms"] / max(tm["fwd_launches"], 1) * tb
     adj_avg_ms = tm["adj_ms"] / max(tm["adj_launches"], 1) * ab
     fwd_kernel = {1: "forward_vec_kernel", 2: "forward2_kernel", 3: "forward3_kernel"}[tb]
-    adj_kernel = "adjoint2_kernel" if ab == 2 else "adjoint_vec_kernel"
+    adj_kernel = {1: "adjoint_vec_kernel", 2: "adjoint2_kernel", 3: "adjoint3_kernel"}[ab]
     fwd_traffic = load_traffic(fwd_kernel, shape, tb) if args.workload == "c3" else None
     adj_traffic = load_traffic(adj_kernel, shape, ab) if args.workload == "c3" else None
 

@@ -9,7 +9,8 @@ Ng), with a fixed relaxation s per problem.
 
 Problem family (unit-spacing preset of ``tomo_config_unit``): an n x n grid,
 n/4 ring transducers at radius 0.39 n fired one at a time, 8 n time steps by
-default, a smooth random velocity perturbation (Gaussian-filtered noise,
+default, a Ricker source of peak frequency fp (default 0.1, 10 cells per
+wavelength), a smooth random velocity perturbation (Gaussian-filtered noise,
 sigma 8 cells, peak +-8 %) plus 4 sharp inclusions inside the central imaging
 mask; measured traces from the FP64 solver, optional additive noise.
 
@@ -60,6 +61,7 @@ class Problem:
     checkpoint: int = -1      # default: every 200 steps (full history below 1000 steps)
     noise: float = 0.0        # additive Gaussian noise, fraction of each trace's RMS
     noise_seed: int = 0
+    fp: float = 0.1           # Ricker peak frequency (0.1: 10 cells per wavelength), delay 1.5 / fp
     device: int = 0
     coverage: bool = True
     engine: Engine = field(init=False, repr=False)
@@ -70,6 +72,8 @@ class Problem:
         self.steps = self.steps or 8 * n
         self.shots = self.shots or n // 4
         cfg = config_unit(n, self.steps + 2)
+        cfg.ricker_fp = self.fp
+        cfg.ricker_t0 = 1.5 / self.fp
         if self.checkpoint < 0:
             self.checkpoint = 200 if self.steps >= 1000 else 0
         cfg.checkpoint_every = self.checkpoint
@@ -193,19 +197,30 @@ def collect_plain(net: Preconditioner, p: Problem, scale: float, iters: int, tar
     iterate the field, the plain step and the coverage (input).  Target:
     "truth" - the true field minus the iterate; "best" - the trajectory's
     iterate of lowest error (over all iterates, the last included) minus the
-    iterate, i.e. where plain SIRT itself gets to.  Returns that best error."""
+    iterate, i.e. where plain SIRT itself gets to; "ahead:K" - the iterate K
+    iterations later (the last one near the end) minus the iterate, so that
+    one learned step stands for K plain ones.  Returns the best error."""
     e = p.engine
     first = net.num_samples()
     best_eps, best_f = None, None
+    ahead = int(target.split(":")[1]) if target.startswith("ahead:") else 0
+    fields = []
     for _ in range(iters):
         e.gradient()
         net.add_sample(e, scale)
         e.update_uniform(scale, UPDATE_MEAN)
         eps = e.error()
+        if ahead:
+            fields.append(e.get_field())
         if best_eps is None or eps < best_eps:
             best_eps, best_f = eps, e.get_field()
     if target == "best":
         net.set_targets(first, iters, best_f)
+    elif ahead:
+        # sample i was taken at iterate i (before its update); iterate i + K
+        # is fields[i + K - 1]
+        for i in range(iters):
+            net.set_targets(first + i, 1, fields[min(i + ahead, iters) - 1])
     elif target != "truth":
         raise ValueError(target)
     return float(best_eps)
@@ -235,7 +250,7 @@ class TrainReport:
 
 def train(net: Preconditioner, n: int, seeds, iters: int, steps: int = 0, shots: int = 0, frac: float = 0.25,
           rounds: int = 1, train_steps: int = 600, batch: int = 8, lr: float = 1e-3, kind: str = "smooth",
-          alpha: float = 1.0, target: str = "best", log=None) -> TrainReport:
+          alpha: float = 1.0, target: str = "best", fp: float = 0.1, log=None) -> TrainReport:
     """Train on plain-SIRT trajectories of phantoms `seeds` (targets as in
     collect_plain), then `rounds` more rounds that add the states of the
     learned iteration on the same phantoms (target: the truth) and continue
@@ -246,7 +261,7 @@ def train(net: Preconditioner, n: int, seeds, iters: int, steps: int = 0, shots:
     scales = {}
     t = time.perf_counter()
     for s in seeds:
-        p = Problem(n, phantom(n, int(s), kind), steps=steps, shots=shots)
+        p = Problem(n, phantom(n, int(s), kind), steps=steps, shots=shots, fp=fp)
         scales[s] = calibrate_scale(p, frac)
         p.reset()
         best = collect_plain(net, p, scales[s], iters, target=target)
@@ -262,7 +277,7 @@ def train(net: Preconditioner, n: int, seeds, iters: int, steps: int = 0, shots:
     for r in range(rounds):
         t = time.perf_counter()
         for s in seeds:
-            p = Problem(n, phantom(n, int(s), kind), steps=steps, shots=shots)
+            p = Problem(n, phantom(n, int(s), kind), steps=steps, shots=shots, fp=fp)
             collect_learned(net, p, scales[s], iters, alpha=alpha)
             p.close()
             if log:
