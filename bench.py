@@ -266,6 +266,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--time-blocking", type=int, default=3, choices=[1, 2, 3],
                     help="forward steps per launch (2, 3: the intermediate levels in registers)")
+    ap.add_argument("--adjoint-blocking", type=int, default=3, choices=[1, 2, 3],
+                    help="most adjoint steps per launch (3: z(k), z(k-1) in registers)")
     ap.add_argument("--shape", default="", help="stencil shape vec,fwd_rows,adj_rows,fwd_depth,adj_depth (default: engine choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--data", default="fp64", choices=["fp64", "fp32"],
@@ -331,6 +333,7 @@ def main():
     if args.graphs:
         e.set_graphs(True)
     e.set_time_blocking(args.time_blocking)
+    e.set_adjoint_blocking(args.adjoint_blocking)
     e.set_transducers(ii, jj, one_based=False)
     e.set_receivers(ii, jj, one_based=False)
     e.set_groups(1)

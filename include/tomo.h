@@ -379,6 +379,14 @@ int tomo_set_time_blocking(tomo_ctx *ctx, int steps);
 int tomo_get_time_blocking(const tomo_ctx *ctx);
 int tomo_get_adjoint_blocking(const tomo_ctx *ctx);
 
+/* Most adjoint steps per launch (1, 2 or 3; default 3).  Where the two-step
+ * adjoint applies (above) and the grid is 256 or 512 cells wide, 3 runs
+ * three adjoint steps per launch: z(k) and z(k-1) stay in registers and
+ * only z(k-1), z(k-2) are stored, 12 instead of 16 bytes per cell-update.
+ * Results are the same bit for bit.  No reference counterpart
+ * (backpropagation1/2 run one step per launch, sirt-mimo.cu:316-327). */
+int tomo_set_adjoint_blocking(tomo_ctx *ctx, int steps);
+
 /* Autotuner: times the vectorised forward and adjoint kernels of this
  * context (its grid, shots per pass, sponge and coverage) over the shapes
  * vec in {4, 2} x rows in {8, 16, 32} x prefetch depth in {1, 2, 4}

@@ -51,14 +51,16 @@ def test_learned_iteration_keeps_only_misfit_decreasing_steps():
 
 
 def test_shipped_network_gains_over_plain_sirt():
-    """Held-out 256^2 phantom (seed 9002), relaxation 0.1 x the
-    misfit-optimal first step: 3 learned iterations then plain ones against
+    """Held-out 256^2 phantom (seed 9002) under the time-to-target study's
+    protocol (source peak frequency 0.05, relaxation 0.3 x the
+    misfit-optimal first step): 3 learned iterations then plain ones against
     plain SIRT, 8 iterations each; the learned run must end at least 5
     points of relative error lower."""
+    import tomo_amd.studies as S
     print("[case] shipped network 256^2", flush=True)
-    p = L.Problem(256, L.phantom(256, 9002))
+    p = L.Problem(256, L.phantom(256, 9002), fp=S.TTT_FP)
     try:
-        sc = L.calibrate_scale(p, 0.1)
+        sc = L.calibrate_scale(p, S.TTT_FRAC)
         p.reset()
         plain = L.run_plain(p, sc, 8)
         p.reset()

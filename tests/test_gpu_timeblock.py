@@ -1,5 +1,5 @@
 """Two and three stencil steps per launch (the intermediate levels held in
-registers; two for the adjoint), against one step per launch: traces, misfits and
+registers; two or three for the adjoint), against one step per launch: traces, misfits and
 gradients must be bitwise equal -- the per-cell arithmetic is the same -- for
 each full-row lane width (nx = 128, 192, 256, 384, 512 -> 2, 3, 4, 6, 8 cells
 per lane; 3 and 6 cells move as single floats and float pairs), with
@@ -17,6 +17,11 @@ from _cases import unit_config, unit_phantom
 from _pkg import tomo
 
 pytestmark = pytest.mark.gpu
+
+
+def adj_steps(n):
+    """Adjoint steps per launch by default: three at widths 256 and 512, else two."""
+    return 3 if n in (256, 512) else 2
 
 
 def engine(n, nt, ckpt, steps, shots=6, coverage=False, receivers=None):
@@ -60,7 +65,7 @@ def test_blocked_launches_match_single_steps(n, nt, ckpt, steps):
     e2, _ = engine(n, nt, ckpt, steps)
     try:
         assert e1.time_blocking() == 1 and e2.time_blocking() == steps
-        assert e1.adjoint_blocking() == 1 and e2.adjoint_blocking() == 2
+        assert e1.adjoint_blocking() == 1 and e2.adjoint_blocking() == adj_steps(n)
         compare(e1, e2, c, 6)
         # the FP32 trace forward (its own rolling ring) agrees too
         np.testing.assert_array_equal(e2.forward(fp64=False), e1.forward(fp64=False))
@@ -77,7 +82,7 @@ def test_many_receivers_on_strip_and_halo_rows(steps):
     e1, c = engine(256, 241, 60, 1, shots=128)
     e2, _ = engine(256, 241, 60, steps, shots=128)
     try:
-        assert e2.time_blocking() == steps and e2.adjoint_blocking() == 2
+        assert e2.time_blocking() == steps and e2.adjoint_blocking() == 3
         compare(e1, e2, c, 128)
     finally:
         e1.close()
@@ -129,9 +134,29 @@ def test_coverage_with_blocked_adjoint(n, nt, ckpt, steps):
     e2, _ = engine(n, nt, ckpt, steps, coverage=True)
     try:
         assert e1.adjoint_blocking() == 1
-        assert e2.time_blocking() == steps and e2.adjoint_blocking() == 2
+        assert e2.time_blocking() == steps and e2.adjoint_blocking() == adj_steps(n)
         compare(e1, e2, c, 6, coverage=True)
         assert e2.get_coverage().max() > 0
+    finally:
+        e1.close()
+        e2.close()
+
+
+@pytest.mark.parametrize("n,nt,ckpt,adj,rows", [(512, 160, 7, 2, 0), (256, 301, 49, 2, 0), (512, 203, 50, 3, 16),
+                                                 (256, 241, 40, 3, 32), (512, 98, 0, 3, 32)])
+def test_adjoint_steps_and_strip_heights(n, nt, ckpt, adj, rows):
+    """The two-step adjoint where three would apply (tomo_set_adjoint_blocking),
+    and the three-step adjoint with 16- and 32-row strips and full history,
+    against one step per launch."""
+    print(f"[case] adjoint steps {adj} rows {rows} n={n} nt={nt} ckpt={ckpt}", flush=True)
+    e1, c = engine(n, nt, ckpt, 1)
+    e2, _ = engine(n, nt, ckpt, 3)
+    try:
+        e2.set_adjoint_blocking(adj)
+        if rows:
+            e2.set_shape(4, 32 if n == 512 else 16, rows, 1, 1)
+        assert e2.adjoint_blocking() == adj
+        compare(e1, e2, c, 6)
     finally:
         e1.close()
         e2.close()

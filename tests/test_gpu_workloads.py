@@ -1,6 +1,7 @@
 """The benchmark workloads against the FP64 oracle, through the C ABI, with
-the kernels each workload runs by default (three forward steps and two
-adjoint steps per launch, checkpoint recompute):
+the kernels each workload runs by default (three forward steps per launch,
+three adjoint steps at widths 256 and 512 and two otherwise, checkpoint
+recompute):
 
   C3  512^2, 4000 steps, checkpoint every 200, smooth +-8 % velocity and four
       inclusions; 8 of the 128 ring shots (every 16th) against all 128 ring
@@ -108,7 +109,7 @@ def test_c3_blocked_kernels_match_oracle_over_4000_steps():
     (Gw, mis_w), G32, mis32 = oracle_gradients(c, f1, bbox, cells, meas)
     say(f"  oracle: {time.perf_counter() - t:.1f} s on {orc.num_threads()} threads")
     with engine(c, si, sj, ii, jj) as e:
-        assert e.time_blocking() == 3 and e.adjoint_blocking() == 2
+        assert e.time_blocking() == 3 and e.adjoint_blocking() == 3
         say("  shape", e.get_shape(), "forward steps per launch", e.time_blocking(),
             "adjoint", e.adjoint_blocking())
         e.set_field(fo)
@@ -199,7 +200,7 @@ def test_c4_sides_match_oracle(n):
     with engine(c, ii, jj, ii, jj) as e:
         say("  shape", e.get_shape(), "forward steps per launch", e.time_blocking(),
             "adjoint", e.adjoint_blocking())
-        assert e.time_blocking() == 3 and e.adjoint_blocking() == 2
+        assert e.time_blocking() == 3 and e.adjoint_blocking() == (3 if n in (256, 512) else 2)
         e.set_field(fo)
         r = rel_l2(e.forward(), meas)
         say(f"  traces rel L2 {r:.2e}")

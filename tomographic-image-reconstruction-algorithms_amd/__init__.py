@@ -131,6 +131,7 @@ SIGNATURES = {
     "tomo_set_time_blocking": (_int, [_vp, _int]),
     "tomo_get_time_blocking": (_int, [_vp]),
     "tomo_get_adjoint_blocking": (_int, [_vp]),
+    "tomo_set_adjoint_blocking": (_int, [_vp, _int]),
     "tomo_autotune": (_int, [_vp, _int, _i32p, _f64p]),
     "tomo_timing_graphs": (_int, [_vp, _f64p]),
     # batch of problems
@@ -468,6 +469,10 @@ class Engine:
     def adjoint_blocking(self) -> int:
         """Adjoint steps per launch in effect (2 where the two-step adjoint applies)."""
         return self._c(lib().tomo_get_adjoint_blocking(self._ctx))
+
+    def set_adjoint_blocking(self, steps: int):
+        """Most adjoint steps per launch (1, 2 or 3)."""
+        self._c(lib().tomo_set_adjoint_blocking(self._ctx, int(steps)))
 
     def autotune(self, force=False):
         """Time the stencil shapes on this context and keep the fastest; returns (shape, (fwd_ms, adj_ms))."""

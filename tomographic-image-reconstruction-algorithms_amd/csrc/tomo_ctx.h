@@ -11,6 +11,10 @@
 #include "tomo.h"
 #include "tomo_kernels.h"
 
+// adjoint levels held per pass: three-step launches write z(k-1), z(k-2)
+// while other strips still read z(k+1), z(k+2) on their halo rows
+constexpr int kAdjRing = 5;
+
 struct TimedSpan {
   hipEvent_t a, b;
   int kind;          // 0 forward, 1 adjoint, 2 graph replay
@@ -77,6 +81,7 @@ struct tomo_ctx {
   // most forward steps per launch (1, 2 or 3; fewer where the larger
   // blockings do not apply)
   int time_block = 3;
+  int adj_block = 3;  // most adjoint steps per launch (tomo_set_adjoint_blocking)
   // per-pixel illumination weights of the updates (0 off, 1 coverage)
   int pixw_mode = 0;
   double pixw_eps = 0.05;

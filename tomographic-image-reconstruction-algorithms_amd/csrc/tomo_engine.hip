@@ -76,6 +76,12 @@ int adjoint2_rows(const tomo_ctx *c) {
   return c->ny >= 512 ? 32 : c->ny >= 256 ? 16 : 8;
 }
 
+// strip height of the three-step adjoint (16 or 32 rows)
+int adjoint3_rows(const tomo_ctx *c) {
+  if (c->shape_req.adj_rows == 16 || c->shape_req.adj_rows == 32) return c->shape_req.adj_rows;
+  return c->ny >= 512 ? 32 : 16;
+}
+
 bool depth_valid(int d) { return d == 0 || d == 1 || d == 2 || d == 4; }
 
 bool shape_valid(int vec, int fr, int ar) {
@@ -304,11 +310,11 @@ int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
     if ((rc = dalloc(c, &c->d_ckpt, ckpt_levels * lvl))) return rc;
     c->ckpt_n = ckpt_levels * lvl;
   }
-  // adjoint levels: a ring of 4 (two-step launches write z(k-1) while other
-  // strips still read z(k+2) on their halo rows)
-  if (need_history && c->z_n < 4 * lvl) {
-    if ((rc = dalloc(c, &c->d_z, 4 * lvl))) return rc;
-    c->z_n = 4 * lvl;
+  // adjoint levels: a ring of 5 (three-step launches write z(k-2) while
+  // other strips still read z(k+2) on their halo rows)
+  if (need_history && c->z_n < kAdjRing * lvl) {
+    if ((rc = dalloc(c, &c->d_z, kAdjRing * lvl))) return rc;
+    c->z_n = kAdjRing * lvl;
   }
   return TOMO_OK;
 }
@@ -339,8 +345,14 @@ int ring_levels(const tomo_ctx *c) { return three_step(c) ? 5 : two_step(c) ? 4 
 // segment (coverage, when on, is summed per checkpoint segment by its own
 // kernel)
 bool two_step_adjoint(const tomo_ctx *c) {
-  return two_step(c) && adjoint2_supported(c->nx, c->cfg.sponge_width) && c->recvseg_ok &&
+  return c->adj_block >= 2 && two_step(c) && adjoint2_supported(c->nx, c->cfg.sponge_width) && c->recvseg_ok &&
          c->cfg.recv_mode == TOMO_RECV_POINTS;
+}
+
+// three adjoint steps per launch: the two-step conditions and a grid the
+// three-step kernel covers
+bool three_step_adjoint(const tomo_ctx *c) {
+  return c->adj_block >= 3 && two_step_adjoint(c) && adjoint3_supported(c->nx, c->cfg.sponge_width);
 }
 
 // location of forward level k during the first (recording) pass (the
@@ -699,7 +711,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   // pass 2: adjoint sweep, newest segment first
   HIPCHK(c, launch_fill_f32(c->d_grad + (size_t)p0 * c->n2, (size_t)np * c->n2, 0.f, c->stream));
   if (c->coverage) HIPCHK(c, launch_fill_f32(c->d_cov + (size_t)p0 * c->n2, (size_t)np * c->n2, 0.f, c->stream));
-  auto Z = [&](int k) { return c->d_z + (size_t)(k % 4) * lvl; };
+  auto Z = [&](int k) { return c->d_z + (size_t)(k % kAdjRing) * lvl; };
   HIPCHK(c, launch_fill_f32(Z(nt - 1), (size_t)np * c->n2, 0.f, c->stream));
   HIPCHK(c, launch_fill_f32(Z(nt), (size_t)np * c->n2, 0.f, c->stream));
   const bool full = full_history(c);
@@ -709,7 +721,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
   StencilShape ash = stencil_shape(c);
   ash.rows = ash.adj_rows;
   ash.depth = ash.adj_depth;
-  const bool two_adj = two_step_adjoint(c);
+  const bool two_adj = two_step_adjoint(c), three_adj = three_step_adjoint(c);
   // with two-step adjoint launches the coverage is summed per segment by its
   // own kernel, so the one-step tails of those passes carry none
   const bool seg_cov = two_adj && c->coverage;
@@ -752,6 +764,22 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
       const int k = k0 + q;
       if (k < 1 || k > nt - 2) {
         --q;
+        continue;
+      }
+      if (three_adj && q >= 2 && k - 2 >= 1) {  // levels k, k-1 and k-2 of this segment
+        Adj3Args t{};
+        t.b = adj2_args(c, p0);
+        t.b.z1 = Z(k + 1);
+        t.b.z2 = Z(k + 2);
+        t.b.zm = Z(k - 1);
+        t.zm2 = Z(k - 2);
+        t.b.u0 = segment_level(c, P, m, k);
+        t.b.um = segment_level(c, P, m, k - 1);
+        t.um2 = segment_level(c, P, m, k - 2);
+        t.b.level = k;
+        HIPCHK(c, launch_adjoint3(t, np, adjoint3_rows(c), c->stream));
+        launches += 3;
+        q -= 3;
         continue;
       }
       if (two_adj && q >= 1 && k - 1 >= 1) {  // levels k and k-1 of this segment
@@ -813,7 +841,8 @@ std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
                          (uintptr_t)c->coverage, (uintptr_t)c->nshots, (uintptr_t)c->nr, (uintptr_t)sh.vec,
                          (uintptr_t)sh.fwd_rows, (uintptr_t)sh.adj_rows, (uintptr_t)sh.fwd_depth,
                          (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c), (uintptr_t)three_step(c),
-                         (uintptr_t)two_step_adjoint(c), (uintptr_t)c->d_recvseg};
+                         (uintptr_t)two_step_adjoint(c), (uintptr_t)three_step_adjoint(c),
+                         (uintptr_t)c->d_recvseg};
   return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
 }
 
@@ -1505,7 +1534,16 @@ int tomo_set_time_blocking(tomo_ctx *c, int steps) {
 int tomo_get_time_blocking(const tomo_ctx *c) {
   return c ? (three_step(c) ? 3 : two_step(c) ? 2 : 1) : TOMO_ERR_ARG;
 }
-int tomo_get_adjoint_blocking(const tomo_ctx *c) { return c ? (two_step_adjoint(c) ? 2 : 1) : TOMO_ERR_ARG; }
+int tomo_get_adjoint_blocking(const tomo_ctx *c) {
+  return c ? (three_step_adjoint(c) ? 3 : two_step_adjoint(c) ? 2 : 1) : TOMO_ERR_ARG;
+}
+
+int tomo_set_adjoint_blocking(tomo_ctx *c, int steps) {
+  if (!c) return TOMO_ERR_ARG;
+  if (steps < 1 || steps > 3) return fail(c, TOMO_ERR_ARG, "adjoint steps per launch must be 1, 2 or 3");
+  c->adj_block = steps;
+  return TOMO_OK;
+}
 
 int tomo_set_pixel_weights(tomo_ctx *c, int mode, double eps) {
   int rc = check_ctx(c);

@@ -204,6 +204,21 @@ struct Adj2Args {
   int sponge_w;
   const float *sp_a, *sp_ab;
 };
+// Three adjoint steps k+1, k+2 -> k -> k-1 -> k-2 in one pass over a strip
+// (same conditions as the two-step adjoint).  z(k) and z(k-1) are kept in
+// registers; only z(k-1) and z(k-2), the levels the next launch starts
+// from, are stored, and the gradient row is read and written once for all
+// three levels (12 instead of 16 bytes per cell-update).  b holds levels
+// k+1, k+2 (z1, z2), k-1 (zm, written) and the forward levels k (u0), k-1
+// (um); b.z0 is not used.
+struct Adj3Args {
+  Adj2Args b;
+  float *zm2;            // adjoint level k-2 (written)
+  const float *um2;      // forward level k-2
+};
+bool adjoint3_supported(int nx, int sponge_w);
+hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, hipStream_t st);
+
 // Coverage of one checkpoint segment's forward levels k = khi .. klo (newest
 // first) into cov[shot0 + s] for the nshots local shots; level k is
 // lev01[k - k0] for k - k0 < 2, else seg + (k - k0) * lstride; shots are

@@ -1427,16 +1427,23 @@ __device__ __forceinline__ Row<VEC> edge_row_lu(const Row<VEC> &E, const Row<VEC
 // the one strip that owns row 1 / ny-2; level k before level k-1, the order
 // of two one-step launches.
 template <int VEC, bool TOP>
-__device__ __forceinline__ void adjoint_edge_row(const Row<VEC> &z, const Row<VEC> &E, const Row<VEC> &I,
-                                                 const Row<VEC> &I2, float *__restrict__ zrow,
-                                                 float *__restrict__ arow, int x0, int nx, float inv_h2) {
-  store_row<VEC>(zrow + x0, z);
+__device__ __forceinline__ void adjoint_edge_corr(const Row<VEC> &z, const Row<VEC> &E, const Row<VEC> &I,
+                                                  const Row<VEC> &I2, float *__restrict__ arow, int x0, int nx,
+                                                  float inv_h2) {
   const Row<VEC> lu = edge_row_lu<VEC, TOP>(E, I, I2, x0, nx, inv_h2);
   const Row<VEC> a = load_row<VEC>(arow + x0, true);
   Row<VEC> out;
 #pragma unroll
   for (int e = 0; e < VEC; ++e) out.e[e] = fused(z.e[e], lu.e[e], a.e[e]);
   store_row<VEC>(arow + x0, out);
+}
+
+template <int VEC, bool TOP>
+__device__ __forceinline__ void adjoint_edge_row(const Row<VEC> &z, const Row<VEC> &E, const Row<VEC> &I,
+                                                 const Row<VEC> &I2, float *__restrict__ zrow,
+                                                 float *__restrict__ arow, int x0, int nx, float inv_h2) {
+  store_row<VEC>(zrow + x0, z);
+  adjoint_edge_corr<VEC, TOP>(z, E, I, I2, arow, x0, nx, inv_h2);
 }
 
 // additive injection at the lane's receiver cell (descriptor d = receiver *
@@ -1640,6 +1647,233 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
     adjoint2_strip<VEC, RY, -1>(A, s_sp, s_acc[threadIdx.y], shot, y0);
   else
     adjoint2_strip<VEC, RY, 1>(A, s_sp, s_acc[threadIdx.y], shot, y0);
+}
+
+// One strip of a three-step adjoint launch, the two-step strip extended by a
+// level.  Each lane walks the rows of its strip in direction D and, at row r,
+// forms z(k) at row r (from the z(k+1) window and z(k+2) row r), z(k-1) at
+// row r-D (from the z(k) window and z(k+1) row r-D) and z(k-2) at row r-2D
+// (from the z(k-1) window and z(k) row r-2D), injecting each level's
+// residual at receivers.  z(k) is formed on two halo rows above and below
+// the strip, z(k-1) on one; z(k-1) and z(k-2) are stored on the strip's own
+// rows (the next launch starts from them), z(k) is not stored.  Each level
+// is correlated with its Lu on the own rows: the gradient row, read once,
+// waits in a lane-private LDS slot (one of three, by row) between the levels
+// and is written once after level k-2.  Rows 0 and ny-1 of every level are
+// copies of rows 1 and ny-2 with their correlations, added level k first.
+// Per three cell-updates: reads z(k+1), z(k+2), u(k), u(k-1), u(k-2), the
+// gradient, writes z(k-1), z(k-2), the gradient (12 B per cell-update).
+// Every cell is the same fixed FMA chain as in the one-step kernels, so the
+// fields and gradients agree with three one-step launches bit for bit.
+template <int VEC, int RY, int D>
+__device__ __forceinline__ void adjoint3_strip(const Adj3Args &A3, const float2 *s_sp,
+                                               float (&stash)[3][VEC][kBlockX], int shot, int y0) {
+  const Adj2Args &A = A3.b;
+  const int nx = A.nx, ny = A.ny;
+  const int yend = min(y0 + RY, ny);
+  const int qa = max(y0, 1), qb = min(yend, ny - 1) - 1;  // own rows of the interior rule
+  const int x0 = threadIdx.x * VEC;
+  const int gs = A.shot0 + shot;
+  const long long so = (long long)shot * A.sstride;
+  const long long n2 = (long long)nx * ny;
+  const float *__restrict__ z1 = A.z1 + so;
+  const float *__restrict__ z2 = A.z2 + so;
+  const float *__restrict__ u0 = A.u0 + so;
+  const float *__restrict__ um = A.um + so;
+  const float *__restrict__ un2 = A3.um2 + so;
+  float *__restrict__ zm = A.zm + so;
+  float *__restrict__ zm2 = A3.zm2 + so;
+  float *__restrict__ acc = A.acc + gs * n2;
+  const float *__restrict__ pw = A.pw;
+  const float *__restrict__ rrk = A.rr + ((long long)gs * A.nt + A.level) * A.nr;  // residuals at level k
+  const float *__restrict__ rrm = rrk - A.nr;                                        // level k-1
+  const float *__restrict__ rrm2 = rrm - A.nr;                                       // level k-2
+  auto ld = [&](const float *__restrict__ base, int y, bool ok) {
+    ok = ok && y >= 0 && y < ny;
+    return load_row<VEC>(base + (ok ? y * nx + x0 : 0), ok);
+  };
+  // rule rows: z(k) on [rs0, re0], z(k-1) on [rs1, re1], z(k-2) on the own rows
+  const int rs0 = max(qa - 2, 1), re0 = min(qb + 2, ny - 2);
+  const int rs1 = max(qa - 1, 1), re1 = min(qb + 1, ny - 2);
+  auto rule0 = [&](int y) { return y >= rs0 && y <= re0; };
+  auto rule1 = [&](int y) { return y >= rs1 && y <= re1; };
+  auto own = [&](int y) { return y >= qa && y <= qb; };
+  auto lrow = [&](int y) { return y >= qa - 1 && y <= qb + 1; };  // u rows of the Laplacians
+  unsigned long long rflags = 0;                                     // rows rs0 .. re0 holding receivers
+#pragma unroll
+  for (int i = 0; i < RY + 4; ++i)
+    if (rs0 + i <= re0 && A.row_has_recv[rs0 + i]) rflags |= 1ull << i;
+  auto desc = [&](int y) {
+    const bool on = rule0(y) && ((rflags >> (y - rs0)) & 1ull);
+    return on ? A.recv_seg[y * kBlockX + threadIdx.x] : -1;
+  };
+  auto resid = [&](int d, const float *__restrict__ r) { return d >= 0 ? r[d / VEC] : 0.f; };
+  auto north = [](const Row<VEC> &behind, const Row<VEC> &ahead) -> const Row<VEC> & {
+    return D > 0 ? behind : ahead;
+  };
+  auto south = [](const Row<VEC> &behind, const Row<VEC> &ahead) -> const Row<VEC> & {
+    return D > 0 ? ahead : behind;
+  };
+  auto slot = [](int y) { return ((y % 3) + 3) % 3; };
+
+  const int r0 = D > 0 ? rs0 : re0, rlast = D > 0 ? qb + 2 : qa - 2;
+  // windows: z(k+1) rows r-D, r, r+D; coefficients r-3D .. r+D; u(k) rows
+  // r-D .. r+D; u(k-1) rows r-2D .. r; u(k-2) rows r-3D .. r-D; z(k+2) and
+  // the gradient at row r; z(k) rows r-2D, r-D and z(k-1) rows r-3D, r-2D
+  // before this row's levels are formed
+  Row<VEC> zb = ld(z1, r0 - D, true), zc = ld(z1, r0, true), za = ld(z1, r0 + D, true);
+  Row<VEC> p3 = ld(pw, r0 - 3 * D, false), pp = ld(pw, r0 - 2 * D, false), pb = ld(pw, r0 - D, true),
+           pc = ld(pw, r0, true), pa = ld(pw, r0 + D, true);
+  Row<VEC> ub = ld(u0, r0 - D, lrow(r0 - D)), uc = ld(u0, r0, lrow(r0)), ua = ld(u0, r0 + D, lrow(r0 + D));
+  Row<VEC> m2 = ld(um, r0 - 2 * D, false), m1 = ld(um, r0 - D, lrow(r0 - D)), m0 = ld(um, r0, lrow(r0));
+  Row<VEC> t3 = ld(un2, r0 - 3 * D, false), t2 = ld(un2, r0 - 2 * D, lrow(r0 - 2 * D)),
+           t1 = ld(un2, r0 - D, lrow(r0 - D));
+  Row<VEC> zz = ld(z2, r0, true);
+  Row<VEC> ac = ld(acc, r0, own(r0));
+  Row<VEC> v2 = ld(acc, 0, false), v1 = v2, w2 = v2, w1 = v2;
+  int dr = desc(r0), dn = desc(r0 + D), dq1 = -1, dq2 = -1;
+  float rk = resid(dr, rrk), rkm = resid(dr, rrm), rkm2 = resid(dr, rrm2);
+  float rq1 = 0.f, rq1b = 0.f, rq2 = 0.f;
+
+#pragma unroll 1
+  for (int r = r0; D > 0 ? r <= rlast : r >= rlast; r += D) {
+    // the next row's inputs are in flight while this row computes
+    const Row<VEC> za2 = ld(z1, r + 2 * D, rule0(r + D));
+    const Row<VEC> pa2 = ld(pw, r + 2 * D, rule0(r + D));
+    const Row<VEC> ua2 = ld(u0, r + 2 * D, lrow(r + 2 * D));
+    const Row<VEC> mn = ld(um, r + D, lrow(r + D));
+    const Row<VEC> tn = ld(un2, r, lrow(r));
+    const Row<VEC> zzn = ld(z2, r + D, rule0(r + D));
+    const Row<VEC> an = ld(acc, r + D, own(r + D));
+    const int d2 = desc(r + 2 * D);
+    // z(k) at row r
+    Row<VEC> v0;
+    if (rule0(r)) {
+      v0 = adjoint_row<VEC>(s_sp, x0, nx, north(pb, pa), north(zb, za), south(pb, pa), south(zb, za), pc, zc, zz, r,
+                            ny, A.sponge_w);
+      inject<VEC>(v0, dr, rk, A.gain);
+      if (own(r)) {
+        const Row<VEC> &un = north(ub, ua), &us = south(ub, ua);
+        const Row<VEC> ao = correlate_row<VEC>(un, uc, us, ac, v0, x0, nx, r, ny, A.inv_h2);
+        if (r == 1) adjoint_edge_corr<VEC, true>(v0, un, uc, us, acc, x0, nx, A.inv_h2);
+        if (r == ny - 2) adjoint_edge_corr<VEC, false>(v0, us, uc, un, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
+        const int sl = slot(r);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) stash[sl][e][threadIdx.x] = ao.e[e];
+      } else {
+        side_copies<VEC>(v0, x0, nx);
+      }
+    } else {
+      v0 = v1;  // row ny-1 (0): the copy of row ny-2 (1)
+    }
+    if (r == (D > 0 ? 1 : ny - 2)) v1 = v0;  // row 0 (ny-1) before the walk reaches it
+    // z(k-1) at row q1 = r-D
+    const int q1 = r - D;
+    Row<VEC> w0;
+    if (rule1(q1)) {
+      w0 = adjoint_row<VEC>(s_sp, x0, nx, north(pp, pc), north(v2, v0), south(pp, pc), south(v2, v0), pb, v1, zb, q1,
+                            ny, A.sponge_w);
+      inject<VEC>(w0, dq1, rq1, A.gain);
+      if (own(q1)) {
+        const int sl = slot(q1);
+        Row<VEC> aq;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) aq.e[e] = stash[sl][e][threadIdx.x];
+        const Row<VEC> &mn_ = north(m2, m0), &ms_ = south(m2, m0);
+        const Row<VEC> ao = correlate_row<VEC>(mn_, m1, ms_, aq, w0, x0, nx, q1, ny, A.inv_h2);
+        store_row<VEC>(zm + q1 * nx + x0, w0);
+        if (q1 == 1) adjoint_edge_row<VEC, true>(w0, mn_, m1, ms_, zm, acc, x0, nx, A.inv_h2);
+        if (q1 == ny - 2)
+          adjoint_edge_row<VEC, false>(w0, ms_, m1, mn_, zm + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) stash[sl][e][threadIdx.x] = ao.e[e];
+      } else {
+        side_copies<VEC>(w0, x0, nx);
+      }
+    } else {
+      w0 = w1;  // row ny-1 (0) of z(k-1): the copy of row ny-2 (1)
+    }
+    if (q1 == (D > 0 ? 1 : ny - 2)) w1 = w0;  // row 0 (ny-1) of z(k-1) before the walk reaches it
+    // z(k-2) at row q2 = r-2D
+    const int q2 = r - 2 * D;
+    if (own(q2)) {
+      const int sl = slot(q2);
+      Row<VEC> aq;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) aq.e[e] = stash[sl][e][threadIdx.x];
+      Row<VEC> x = adjoint_row<VEC>(s_sp, x0, nx, north(p3, pb), north(w2, w0), south(p3, pb), south(w2, w0), pp, w1,
+                                    v2, q2, ny, A.sponge_w);
+      inject<VEC>(x, dq2, rq2, A.gain);
+      const Row<VEC> &tn_ = north(t3, t1), &ts_ = south(t3, t1);
+      const Row<VEC> ao = correlate_row<VEC>(tn_, t2, ts_, aq, x, x0, nx, q2, ny, A.inv_h2);
+      store_row<VEC>(zm2 + q2 * nx + x0, x);
+      store_row<VEC>(acc + q2 * nx + x0, ao);
+      if (q2 == 1) adjoint_edge_row<VEC, true>(x, tn_, t2, ts_, zm2, acc, x0, nx, A.inv_h2);
+      if (q2 == ny - 2)
+        adjoint_edge_row<VEC, false>(x, ts_, t2, tn_, zm2 + (ny - 1) * nx, acc + (ny - 1) * nx, x0, nx, A.inv_h2);
+    }
+    // residuals of row r+D (its descriptor was read a row ago)
+    const float rkn = resid(dn, rrk), rkmn = resid(dn, rrm), rkm2n = resid(dn, rrm2);
+    ac = an;
+    v2 = v1;
+    v1 = v0;
+    w2 = w1;
+    w1 = w0;
+    p3 = pp;
+    pp = pb;
+    pb = pc;
+    pc = pa;
+    pa = pa2;
+    zb = zc;
+    zc = za;
+    za = za2;
+    ub = uc;
+    uc = ua;
+    ua = ua2;
+    m2 = m1;
+    m1 = m0;
+    m0 = mn;
+    t3 = t2;
+    t2 = t1;
+    t1 = tn;
+    zz = zzn;
+    dq2 = dq1;
+    dq1 = dr;
+    dr = dn;
+    dn = d2;
+    rq2 = rq1b;
+    rq1b = rkm2;
+    rq1 = rkm;
+    rk = rkn;
+    rkm = rkmn;
+    rkm2 = rkm2n;
+  }
+}
+
+// Three adjoint steps per launch over strips of RY rows; even strips walk
+// down, odd strips up (as the two-step kernel).
+template <int VEC, int RY>
+__global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_eu(VEC > 4 ? 1 : 2))) void adjoint3_kernel(
+    Adj3Args A3) {
+  const Adj2Args &A = A3.b;
+  const int ny = A.ny;
+  __shared__ float2 s_sp[kSpongeLds];
+  // the gradient rows between their levels, in lane-private LDS slots
+  // ([wave][slot][cell][lane]: conflict-free)
+  __shared__ float s_acc[kBlockY][3][VEC][kBlockX];
+  for (int i = threadIdx.y * kBlockX + threadIdx.x; i < A.sponge_w; i += kBlockX * kBlockY)
+    s_sp[i] = make_float2(A.sp_a[i], A.sp_ab[i]);
+  __syncthreads();
+  const int lb = xcd_logical_block();
+  const int shot = lb / A.blocks_per_shot;
+  const int strip = (lb % A.blocks_per_shot) * kBlockY + wave_row();
+  const int y0 = strip * RY;
+  const int yend = min(y0 + RY, ny);
+  if (y0 >= ny || max(y0, 1) > min(yend, ny - 1) - 1) return;  // whole wavefront
+  if (strip & 1)
+    adjoint3_strip<VEC, RY, -1>(A3, s_sp, s_acc[threadIdx.y], shot, y0);
+  else
+    adjoint3_strip<VEC, RY, 1>(A3, s_sp, s_acc[threadIdx.y], shot, y0);
 }
 
 
@@ -2431,6 +2665,30 @@ hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st) {
     return hipErrorInvalidValue;
   }
 #undef TOMO_ADJ2
+  return hipGetLastError();
+}
+
+bool adjoint3_supported(int nx, int sponge_w) {
+  return sponge_w > 0 && sponge_w <= kSpongeLds && (nx == 256 || nx == 512);
+}
+
+hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, hipStream_t st) {
+  Adj2Args &b = a.b;
+  if (!adjoint3_supported(b.nx, b.sponge_w)) return hipErrorInvalidValue;
+  const int vec = b.nx / kBlockX;
+  const int ry = rows == 16 ? 16 : 32;
+  const int strips = (b.ny + ry - 1) / ry;
+  b.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
+  const dim3 grid(b.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
+#define TOMO_ADJ3(V)                                                                  \
+  if (ry == 16) adjoint3_kernel<V, 16><<<grid, block, 0, st>>>(a);                    \
+  else adjoint3_kernel<V, 32><<<grid, block, 0, st>>>(a);
+  if (vec == 8) {
+    TOMO_ADJ3(8)
+  } else {
+    TOMO_ADJ3(4)
+  }
+#undef TOMO_ADJ3
   return hipGetLastError();
 }
 
