@@ -251,6 +251,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
     ap.add_argument("--c4-threads", type=int, default=4,
                     help="host threads stepping the C4 phantoms concurrently (1: one after the other)")
+    ap.add_argument("--c4-passes", type=int, default=0,
+                    help="C4 shots per pass (0: all shots in one pass, the phantoms of each host thread sharing "
+                         "one wavefield workspace)")
     ap.add_argument("--graphs", action="store_true", help="replay each gradient pass as a captured HIP graph")
     ap.add_argument("--study", choices=["c2", "ttt", "train", "finetune"], default=None,
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
@@ -464,24 +467,16 @@ def run_c4(args):
     iteration of every phantom.  The phantoms are dealt by cost (n^4) to
     --c4-threads host threads, each stepping its contexts one after the
     other on their own streams, so launches of different phantoms overlap
-    on the GPU (the one-launch-per-level batch of tomo_batch is switched off
-    at this size: its one full-size run ended in a host SIGSEGV whose cause
-    is not established)."""
+    on the GPU.  With --c4-passes 0 (the default) every phantom runs all its
+    shots in one pass, the phantoms of a thread sharing one wavefield
+    workspace.  (tomo_batch refuses a program of this size: about 260k
+    launches per gradient.)"""
     from concurrent.futures import ThreadPoolExecutor
     log = (lambda m: print("[bench]", m, file=sys.stderr, flush=True))
     t = time.perf_counter()
-    engines = W.c4_problems(log=log)
+    engines = W.c4_problems(log=log, passes=args.c4_passes)
     t_data = time.perf_counter() - t
     sides = W.c4_sides()
-    shapes = {}
-    for e, n in zip(engines, sides):
-        shapes[n] = e.autotune()[0]
-    log(f"data for {len(engines)} phantoms in {t_data:.1f} s; shapes per side {shapes}")
-    scales = []
-    for e, n in zip(engines, sides):
-        e.gradient()
-        g = np.stack([e.get_gradient(s) for s in range(e.num_shots)]).astype(np.float64).sum(0) / e.num_shots
-        scales.append(float(1e-3 / max(np.abs(g).max(), 1e-30)))
 
     # deal the phantoms to the threads, largest first, each to the least loaded
     nthreads = max(1, args.c4_threads)
@@ -491,6 +486,24 @@ def run_c4(args):
         t_min = min(range(nthreads), key=lambda t: load[t])
         groups[t_min].append(i)
         load[t_min] += float(sides[i]) ** 4
+    # each thread steps its phantoms one after another: with all shots in one
+    # pass they share one wavefield workspace, sized for the largest of them
+    workspaces = []
+    if args.c4_passes == 0:
+        for g in groups:
+            ws = tomo.Workspace(0)
+            for i in g:
+                ws.attach(engines[i])
+            workspaces.append(ws)
+    shapes = {}
+    for e, n in zip(engines, sides):
+        shapes[n] = e.autotune()[0]
+    log(f"data for {len(engines)} phantoms in {t_data:.1f} s; shapes per side {shapes}")
+    scales = []
+    for e, n in zip(engines, sides):
+        e.gradient()
+        g = np.stack([e.get_gradient(s) for s in range(e.num_shots)]).astype(np.float64).sum(0) / e.num_shots
+        scales.append(float(1e-3 / max(np.abs(g).max(), 1e-30)))
 
     def step_group(idx):
         for i in idx:
@@ -531,8 +544,12 @@ def run_c4(args):
         "data": "synthetic (seeded phantoms; measured traces from the engine's FP64 solver)",
         "config": {
             "workload": "uneven set: 64 phantoms, sides from {128,192,256,384,512}, n/4 ring shots, 8n steps, "
-                        "checkpoint every 128, passes of 32 shots, one weighted SIRT iteration of every phantom "
-                        f"per step, phantoms stepped by {nthreads} host threads on their own streams",
+                        "checkpoint every 128, "
+                        + (f"passes of {args.c4_passes} shots" if args.c4_passes else
+                           "all shots in one pass (one wavefield workspace per host thread)")
+                        + f", one weighted SIRT iteration of every phantom per step, phantoms stepped by {nthreads} "
+                        "host threads on their own streams",
+            "shots_per_pass": args.c4_passes,
             "host_threads": nthreads,
             "sides": sides, "shapes": {str(k): list(v) for k, v in shapes.items()},
             "cell_updates_per_step": cells / args.steps, "mean_eps_after_steps": eps,
@@ -544,6 +561,8 @@ def run_c4(args):
     }
     for e in engines:
         e.close()
+    for ws in workspaces:
+        ws.close()
     print(json.dumps(out), flush=True)
 
 

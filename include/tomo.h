@@ -419,6 +419,25 @@ int tomo_timing_graphs(tomo_ctx *ctx, double *out3);
 /* out5 = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth} in effect */
 int tomo_get_shape(const tomo_ctx *ctx, int *out5);
 
+/* -------------------------------------------------------------- workspace
+ * Wavefield storage (checkpoint level pairs, the segment being replayed and
+ * the adjoint ring: the bulk of a context's device memory) shared by
+ * contexts that are never used at the same time, e.g. contexts stepped one
+ * after another by one host thread.  The storage holds nothing between
+ * calls; it grows to the largest need among its contexts, so a set of
+ * problems of different sizes can run every shot in one pass
+ * (shots_per_pass = 0) in the memory of its largest member.  No reference
+ * counterpart (the reference allocates per problem,
+ * sirt-mimo-norm-mem.cu:262-282). */
+typedef struct tomo_workspace tomo_workspace;
+
+int tomo_workspace_create(int device, tomo_workspace **out);
+/* TOMO_ERR_STATE while contexts still use it (destroy or detach them first) */
+int tomo_workspace_destroy(tomo_workspace *ws);
+/* attach ctx to ws (NULL: back to storage of its own); calls on contexts
+ * that share a workspace must not overlap; batched contexts cannot share one */
+int tomo_set_workspace(tomo_ctx *ctx, tomo_workspace *ws);
+
 /* ------------------------------------------------------------------ batch
  * Many independent problems stepped together (uneven batches of phantoms
  * of different grid, shot and step counts).  No reference counterpart: the

@@ -13,10 +13,10 @@ pytestmark = pytest.mark.gpu
 SIZES = [(64, 16, 512), (96, 24, 768), (128, 32, 1024)]  # n, shots (n/4), steps (8n)
 
 
-def problem(n, shots, steps, seed, coverage):
+def problem(n, shots, steps, seed, coverage, passes=12):
     c = unit_config(n, steps + 2)
     c.checkpoint_every = 100
-    c.shots_per_pass = 12
+    c.shots_per_pass = passes
     ii, jj = tomo.transducers_ring(n, n, shots, 0.39 * n)
     fo = unit_phantom(c, seed=seed)
     e = tomo.Engine(c, device=0)
@@ -114,3 +114,40 @@ def test_batch_refuses_oversized_program():
             tomo.Batch([e])
     finally:
         e.close()
+
+
+def test_shared_workspace_matches_own_storage():
+    """Three problems of different sizes share one wavefield workspace (every
+    shot in one pass) and are stepped one after another, twice, the middle
+    one by graph replay: the pool grows to the largest need and every
+    gradient equals the same problem's on storage of its own, bit for bit.
+    A batch refuses contexts with a workspace, and the workspace refuses to
+    be destroyed while contexts use it."""
+    print("[case] shared workspace", flush=True)
+    sizes = [(64, 16, 512), (128, 32, 1024), (96, 24, 768)]
+    alone = [problem(n, s, t, 20 + i, False, passes=0) for i, (n, s, t) in enumerate(sizes)]
+    shared = [problem(n, s, t, 20 + i, False, passes=0) for i, (n, s, t) in enumerate(sizes)]
+    ws = tomo.Workspace(0)
+    try:
+        shared[1].set_graphs(True)
+        alone[1].set_graphs(True)
+        for e in shared:
+            ws.attach(e)
+        want = [e.gradient() for e in alone]
+        ref = [[e.get_gradient(g) for g in range(s)] for e, (n, s, t) in zip(alone, sizes)]
+        for rep in range(2):
+            for i, e in enumerate(shared):
+                assert e.gradient() == want[i], (rep, i)
+                for g in range(sizes[i][1]):
+                    np.testing.assert_array_equal(e.get_gradient(g), ref[i][g])
+        with pytest.raises(tomo.TomoError, match="workspace"):
+            tomo.Batch(shared)
+        with pytest.raises(tomo.TomoError, match="still use"):
+            ws.close()
+        for e in shared:
+            e.detach_workspace()
+        assert shared[0].gradient() == want[0]   # back on storage of its own
+        ws.close()
+    finally:
+        for e in alone + shared:
+            e.close()

@@ -142,19 +142,21 @@ def test_coverage_with_blocked_adjoint(n, nt, ckpt, steps):
         e2.close()
 
 
-@pytest.mark.parametrize("n,nt,ckpt,adj,rows", [(512, 160, 7, 2, 0), (256, 301, 49, 2, 0), (512, 203, 50, 3, 16),
-                                                 (256, 241, 40, 3, 32), (512, 98, 0, 3, 32)])
-def test_adjoint_steps_and_strip_heights(n, nt, ckpt, adj, rows):
+@pytest.mark.parametrize("n,nt,ckpt,adj,rows,depth", [(512, 160, 7, 2, 0, 1), (256, 301, 49, 2, 0, 1),
+                                                       (512, 203, 50, 3, 16, 1), (256, 241, 40, 3, 32, 1),
+                                                       (512, 98, 0, 3, 32, 1), (512, 203, 50, 3, 32, 2),
+                                                       (256, 160, 7, 3, 16, 2)])
+def test_adjoint_steps_and_strip_heights(n, nt, ckpt, adj, rows, depth):
     """The two-step adjoint where three would apply (tomo_set_adjoint_blocking),
-    and the three-step adjoint with 16- and 32-row strips and full history,
-    against one step per launch."""
-    print(f"[case] adjoint steps {adj} rows {rows} n={n} nt={nt} ckpt={ckpt}", flush=True)
+    and the three-step adjoint with 16- and 32-row strips, full history and
+    one or two rows of read-ahead, against one step per launch."""
+    print(f"[case] adjoint steps {adj} rows {rows} depth {depth} n={n} nt={nt} ckpt={ckpt}", flush=True)
     e1, c = engine(n, nt, ckpt, 1)
     e2, _ = engine(n, nt, ckpt, 3)
     try:
         e2.set_adjoint_blocking(adj)
         if rows:
-            e2.set_shape(4, 32 if n == 512 else 16, rows, 1, 1)
+            e2.set_shape(4, 32 if n == 512 else 16, rows, 1, depth)
         assert e2.adjoint_blocking() == adj
         compare(e1, e2, c, 6)
     finally:

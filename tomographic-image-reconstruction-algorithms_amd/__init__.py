@@ -132,6 +132,9 @@ SIGNATURES = {
     "tomo_get_time_blocking": (_int, [_vp]),
     "tomo_get_adjoint_blocking": (_int, [_vp]),
     "tomo_set_adjoint_blocking": (_int, [_vp, _int]),
+    "tomo_workspace_create": (_int, [_int, ctypes.POINTER(_vp)]),
+    "tomo_workspace_destroy": (_int, [_vp]),
+    "tomo_set_workspace": (_int, [_vp, _vp]),
     "tomo_autotune": (_int, [_vp, _int, _i32p, _f64p]),
     "tomo_timing_graphs": (_int, [_vp, _f64p]),
     # batch of problems
@@ -470,6 +473,10 @@ class Engine:
         """Adjoint steps per launch in effect (2 where the two-step adjoint applies)."""
         return self._c(lib().tomo_get_adjoint_blocking(self._ctx))
 
+    def detach_workspace(self):
+        """Back to wavefield storage of the engine's own."""
+        self._c(lib().tomo_set_workspace(self._ctx, None))
+
     def set_adjoint_blocking(self, steps: int):
         """Most adjoint steps per launch (1, 2 or 3)."""
         self._c(lib().tomo_set_adjoint_blocking(self._ctx, int(steps)))
@@ -493,6 +500,36 @@ class Engine:
         out = np.zeros(5, np.int32)
         self._c(lib().tomo_get_shape(self._ctx, out))
         return tuple(int(v) for v in out)
+
+
+# ----------------------------------------------------------- workspace
+
+class Workspace:
+    """Wavefield storage shared by engines that are never used at the same
+    time (see tomo_set_workspace); destroy it after those engines."""
+
+    def __init__(self, device: int = 0):
+        self._w = _vp()
+        st = lib().tomo_workspace_create(int(device), ctypes.byref(self._w))
+        if st < 0:
+            raise TomoError(_STATUS.get(st, str(st)))
+
+    def attach(self, engine: "Engine"):
+        engine._c(lib().tomo_set_workspace(engine._ctx, self._w))
+
+    def close(self):
+        if self._w:
+            st = lib().tomo_workspace_destroy(self._w)
+            if st < 0:
+                detail = lib().tomo_last_error(None)
+                raise TomoError(_STATUS.get(st, str(st)) + (": " + detail.decode(errors="replace") if detail else ""))
+            self._w = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # --------------------------------------------------------------- batch

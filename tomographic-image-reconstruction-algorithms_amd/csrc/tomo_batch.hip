@@ -356,6 +356,7 @@ int tomo_batch_create(tomo_ctx *const *ctxs, int n, tomo_batch **out) {
     else if (c->nshots <= 0 || c->nr <= 0) rc = bfail(b, TOMO_ERR_STATE, "context without shots or receivers");
     else if (!c->have_meas) rc = bfail(b, TOMO_ERR_STATE, "context without measured traces");
     else if (c->cfg.sponge_width <= 0) rc = bfail(b, TOMO_ERR_ARG, "batched contexts use the sponge preset");
+    else if (c->ws) rc = bfail(b, TOMO_ERR_ARG, "batched contexts run together: they cannot share a workspace");
     else if (context_shape(c).vec < 2) rc = bfail(b, TOMO_ERR_ARG, "grid too small for the vectorised kernels");
     else b->ctx.push_back(c);
   }

@@ -31,6 +31,16 @@ struct GraphEntry {
   double launches[2] = {0, 0}, cells[2] = {0, 0};
 };
 
+// Wavefield storage shared by contexts that are never used at the same time
+// (tomo_workspace_create, tomo_set_workspace): it grows to the largest need
+// among them.
+struct tomo_workspace {
+  int device = 0;
+  float *d_ckpt = nullptr, *d_seg = nullptr, *d_z = nullptr;
+  size_t ckpt_n = 0, seg_n = 0, z_n = 0;
+  int users = 0;
+};
+
 struct tomo_ctx {
   tomo_config cfg{};
   int device = 0;
@@ -63,9 +73,10 @@ struct tomo_ctx {
   double *d_misfit = nullptr;
   double *d_scratch = nullptr;
   size_t scratch_n = 0;
-  // wavefield pool
+  // wavefield pool: the context's own, or views of a shared workspace (ws)
   float *d_ckpt = nullptr, *d_seg = nullptr, *d_z = nullptr;
   size_t ckpt_n = 0, seg_n = 0, z_n = 0;
+  tomo_workspace *ws = nullptr;
   double *d_ring64 = nullptr;
   size_t ring64_n = 0;
   // state

@@ -76,6 +76,10 @@ int adjoint2_rows(const tomo_ctx *c) {
   return c->ny >= 512 ? 32 : c->ny >= 256 ? 16 : 8;
 }
 
+// read-ahead of the three-step adjoint (1 or 2 rows; tomo_set_shape's
+// adjoint depth)
+int adjoint3_depth(const tomo_ctx *c) { return c->shape_req.adj_depth == 2 ? 2 : 1; }
+
 // strip height of the three-step adjoint (16 or 32 rows)
 int adjoint3_rows(const tomo_ctx *c) {
   if (c->shape_req.adj_rows == 16 || c->shape_req.adj_rows == 32) return c->shape_req.adj_rows;
@@ -301,22 +305,34 @@ int ensure_wavefields(tomo_ctx *c, int P, bool need_history) {
     seg_levels = (size_t)C;
     ckpt_levels = 2 * (size_t)((c->nt + C - 1) / C);
   }
-  int rc;
-  if (c->seg_n < seg_levels * lvl) {
-    if ((rc = dalloc(c, &c->d_seg, seg_levels * lvl))) return rc;
-    c->seg_n = seg_levels * lvl;
-  }
-  if (c->ckpt_n < ckpt_levels * lvl) {
-    if ((rc = dalloc(c, &c->d_ckpt, ckpt_levels * lvl))) return rc;
-    c->ckpt_n = ckpt_levels * lvl;
-  }
   // adjoint levels: a ring of 5 (three-step launches write z(k-2) while
   // other strips still read z(k+2) on their halo rows)
-  if (need_history && c->z_n < kAdjRing * lvl) {
-    if ((rc = dalloc(c, &c->d_z, kAdjRing * lvl))) return rc;
-    c->z_n = kAdjRing * lvl;
+  const size_t z_levels = need_history ? kAdjRing : 0;
+  // the storage grows, never shrinks: a shared workspace keeps the largest
+  // need of its contexts, and each context views it
+  tomo_workspace own{};
+  tomo_workspace *w = c->ws;
+  if (!w) {
+    own.d_ckpt = c->d_ckpt; own.d_seg = c->d_seg; own.d_z = c->d_z;
+    own.ckpt_n = c->ckpt_n; own.seg_n = c->seg_n; own.z_n = c->z_n;
+    w = &own;
   }
-  return TOMO_OK;
+  int rc = TOMO_OK;
+  if (w->seg_n < seg_levels * lvl) {
+    if ((rc = dalloc(c, &w->d_seg, seg_levels * lvl)) == TOMO_OK) w->seg_n = seg_levels * lvl;
+    else w->seg_n = 0;
+  }
+  if (!rc && w->ckpt_n < ckpt_levels * lvl) {
+    if ((rc = dalloc(c, &w->d_ckpt, ckpt_levels * lvl)) == TOMO_OK) w->ckpt_n = ckpt_levels * lvl;
+    else w->ckpt_n = 0;
+  }
+  if (!rc && w->z_n < z_levels * lvl) {
+    if ((rc = dalloc(c, &w->d_z, z_levels * lvl)) == TOMO_OK) w->z_n = z_levels * lvl;
+    else w->z_n = 0;
+  }
+  c->d_ckpt = w->d_ckpt; c->d_seg = w->d_seg; c->d_z = w->d_z;
+  c->ckpt_n = w->ckpt_n; c->seg_n = w->seg_n; c->z_n = w->z_n;
+  return rc;
 }
 
 // two forward steps per launch in the gradient passes: the sponge preset,
@@ -777,7 +793,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
         t.b.um = segment_level(c, P, m, k - 1);
         t.um2 = segment_level(c, P, m, k - 2);
         t.b.level = k;
-        HIPCHK(c, launch_adjoint3(t, np, adjoint3_rows(c), c->stream));
+        HIPCHK(c, launch_adjoint3(t, np, adjoint3_rows(c), adjoint3_depth(c), c->stream));
         launches += 3;
         q -= 3;
         continue;
@@ -1073,6 +1089,10 @@ void tomo_destroy(tomo_ctx *c) {
   dfree(c->d_recvseg);
   dfree(c->d_spa); dfree(c->d_spab); dfree(c->d_spa64); dfree(c->d_spab64);
   dfree(c->d_meas); dfree(c->d_rr); dfree(c->d_grad); dfree(c->d_cov); dfree(c->d_misfit); dfree(c->d_pixw);
+  if (c->ws) {  // the workspace's storage outlives the context
+    c->ws->users--;
+    c->d_ckpt = c->d_seg = c->d_z = nullptr;
+  }
   dfree(c->d_scratch); dfree(c->d_ckpt); dfree(c->d_seg); dfree(c->d_z); dfree(c->d_ring64);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1542,6 +1562,54 @@ int tomo_set_adjoint_blocking(tomo_ctx *c, int steps) {
   if (!c) return TOMO_ERR_ARG;
   if (steps < 1 || steps > 3) return fail(c, TOMO_ERR_ARG, "adjoint steps per launch must be 1, 2 or 3");
   c->adj_block = steps;
+  return TOMO_OK;
+}
+
+int tomo_workspace_create(int device, tomo_workspace **out) {
+  if (!out) return TOMO_ERR_ARG;
+  *out = nullptr;
+  int dev = device;
+  if (dev < 0 && hipGetDevice(&dev) != hipSuccess) {
+    tomo_host::set_create_error("tomo_workspace_create: no usable GPU");
+    return TOMO_ERR_NODEV;
+  }
+  tomo_workspace *w = new tomo_workspace;
+  w->device = dev;
+  *out = w;
+  return TOMO_OK;
+}
+
+int tomo_workspace_destroy(tomo_workspace *w) {
+  if (!w) return TOMO_ERR_ARG;
+  if (w->users > 0) {
+    tomo_host::set_create_error("tomo_workspace_destroy: contexts still use the workspace");
+    return TOMO_ERR_STATE;
+  }
+  (void)hipSetDevice(w->device);
+  dfree(w->d_ckpt);
+  dfree(w->d_seg);
+  dfree(w->d_z);
+  delete w;
+  return TOMO_OK;
+}
+
+int tomo_set_workspace(tomo_ctx *c, tomo_workspace *w) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (w == c->ws) return TOMO_OK;
+  if (w && w->device != c->device) return fail(c, TOMO_ERR_ARG, "workspace and context are on different devices");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->ws) {
+    c->ws->users--;
+    c->d_ckpt = c->d_seg = c->d_z = nullptr;  // the old workspace keeps its storage
+  } else {
+    dfree(c->d_ckpt);  // the context's own storage
+    dfree(c->d_seg);
+    dfree(c->d_z);
+  }
+  c->ckpt_n = c->seg_n = c->z_n = 0;
+  c->ws = w;
+  if (w) w->users++;
   return TOMO_OK;
 }
 

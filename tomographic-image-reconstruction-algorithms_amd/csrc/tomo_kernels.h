@@ -217,7 +217,8 @@ struct Adj3Args {
   const float *um2;      // forward level k-2
 };
 bool adjoint3_supported(int nx, int sponge_w);
-hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, hipStream_t st);
+// depth: rows whose loads are in flight ahead of the row being formed (1, 2)
+hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, int depth, hipStream_t st);
 
 // Coverage of one checkpoint segment's forward levels k = khi .. klo (newest
 // first) into cov[shot0 + s] for the nshots local shots; level k is

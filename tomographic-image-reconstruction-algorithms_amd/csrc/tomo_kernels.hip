@@ -1665,7 +1665,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
 // gradient, writes z(k-1), z(k-2), the gradient (12 B per cell-update).
 // Every cell is the same fixed FMA chain as in the one-step kernels, so the
 // fields and gradients agree with three one-step launches bit for bit.
-template <int VEC, int RY, int D>
+template <int VEC, int RY, int D, int PD>
 __device__ __forceinline__ void adjoint3_strip(const Adj3Args &A3, const float2 *s_sp,
                                                float (&stash)[3][VEC][kBlockX], int shot, int y0) {
   const Adj2Args &A = A3.b;
@@ -1735,16 +1735,48 @@ __device__ __forceinline__ void adjoint3_strip(const Adj3Args &A3, const float2 
   float rk = resid(dr, rrk), rkm = resid(dr, rrm), rkm2 = resid(dr, rrm2);
   float rq1 = 0.f, rq1b = 0.f, rq2 = 0.f;
 
+  // read-ahead of PD rows: with PD = 2 the next row's inputs were issued a
+  // row earlier (the *_q registers) and those of the row after are issued
+  // while this row computes
+  Row<VEC> za_q, pa_q, ua_q, mn_q, tn_q, zzn_q, an_q;
+  if constexpr (PD == 2) {
+    za_q = ld(z1, r0 + 2 * D, rule0(r0 + D));
+    pa_q = ld(pw, r0 + 2 * D, rule0(r0 + D));
+    ua_q = ld(u0, r0 + 2 * D, lrow(r0 + 2 * D));
+    mn_q = ld(um, r0 + D, lrow(r0 + D));
+    tn_q = ld(un2, r0, lrow(r0));
+    zzn_q = ld(z2, r0 + D, rule0(r0 + D));
+    an_q = ld(acc, r0 + D, own(r0 + D));
+  }
+
 #pragma unroll 1
   for (int r = r0; D > 0 ? r <= rlast : r >= rlast; r += D) {
     // the next row's inputs are in flight while this row computes
-    const Row<VEC> za2 = ld(z1, r + 2 * D, rule0(r + D));
-    const Row<VEC> pa2 = ld(pw, r + 2 * D, rule0(r + D));
-    const Row<VEC> ua2 = ld(u0, r + 2 * D, lrow(r + 2 * D));
-    const Row<VEC> mn = ld(um, r + D, lrow(r + D));
-    const Row<VEC> tn = ld(un2, r, lrow(r));
-    const Row<VEC> zzn = ld(z2, r + D, rule0(r + D));
-    const Row<VEC> an = ld(acc, r + D, own(r + D));
+    Row<VEC> za2, pa2, ua2, mn, tn, zzn, an;
+    if constexpr (PD == 2) {
+      za2 = za_q;
+      pa2 = pa_q;
+      ua2 = ua_q;
+      mn = mn_q;
+      tn = tn_q;
+      zzn = zzn_q;
+      an = an_q;
+      za_q = ld(z1, r + 3 * D, rule0(r + 2 * D));
+      pa_q = ld(pw, r + 3 * D, rule0(r + 2 * D));
+      ua_q = ld(u0, r + 3 * D, lrow(r + 3 * D));
+      mn_q = ld(um, r + 2 * D, lrow(r + 2 * D));
+      tn_q = ld(un2, r + D, lrow(r + D));
+      zzn_q = ld(z2, r + 2 * D, rule0(r + 2 * D));
+      an_q = ld(acc, r + 2 * D, own(r + 2 * D));
+    } else {
+      za2 = ld(z1, r + 2 * D, rule0(r + D));
+      pa2 = ld(pw, r + 2 * D, rule0(r + D));
+      ua2 = ld(u0, r + 2 * D, lrow(r + 2 * D));
+      mn = ld(um, r + D, lrow(r + D));
+      tn = ld(un2, r, lrow(r));
+      zzn = ld(z2, r + D, rule0(r + D));
+      an = ld(acc, r + D, own(r + D));
+    }
     const int d2 = desc(r + 2 * D);
     // z(k) at row r
     Row<VEC> v0;
@@ -1852,7 +1884,7 @@ __device__ __forceinline__ void adjoint3_strip(const Adj3Args &A3, const float2 
 
 // Three adjoint steps per launch over strips of RY rows; even strips walk
 // down, odd strips up (as the two-step kernel).
-template <int VEC, int RY>
+template <int VEC, int RY, int PD>
 __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_eu(VEC > 4 ? 1 : 2))) void adjoint3_kernel(
     Adj3Args A3) {
   const Adj2Args &A = A3.b;
@@ -1871,9 +1903,9 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
   const int yend = min(y0 + RY, ny);
   if (y0 >= ny || max(y0, 1) > min(yend, ny - 1) - 1) return;  // whole wavefront
   if (strip & 1)
-    adjoint3_strip<VEC, RY, -1>(A3, s_sp, s_acc[threadIdx.y], shot, y0);
+    adjoint3_strip<VEC, RY, -1, PD>(A3, s_sp, s_acc[threadIdx.y], shot, y0);
   else
-    adjoint3_strip<VEC, RY, 1>(A3, s_sp, s_acc[threadIdx.y], shot, y0);
+    adjoint3_strip<VEC, RY, 1, PD>(A3, s_sp, s_acc[threadIdx.y], shot, y0);
 }
 
 
@@ -2672,7 +2704,7 @@ bool adjoint3_supported(int nx, int sponge_w) {
   return sponge_w > 0 && sponge_w <= kSpongeLds && (nx == 256 || nx == 512);
 }
 
-hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, hipStream_t st) {
+hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, int depth, hipStream_t st) {
   Adj2Args &b = a.b;
   if (!adjoint3_supported(b.nx, b.sponge_w)) return hipErrorInvalidValue;
   const int vec = b.nx / kBlockX;
@@ -2681,8 +2713,10 @@ hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, hipStream_t st) {
   b.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
   const dim3 grid(b.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
 #define TOMO_ADJ3(V)                                                                  \
-  if (ry == 16) adjoint3_kernel<V, 16><<<grid, block, 0, st>>>(a);                    \
-  else adjoint3_kernel<V, 32><<<grid, block, 0, st>>>(a);
+  if (ry == 16 && depth == 2) adjoint3_kernel<V, 16, 2><<<grid, block, 0, st>>>(a);   \
+  else if (ry == 16) adjoint3_kernel<V, 16, 1><<<grid, block, 0, st>>>(a);            \
+  else if (depth == 2) adjoint3_kernel<V, 32, 2><<<grid, block, 0, st>>>(a);          \
+  else adjoint3_kernel<V, 32, 1><<<grid, block, 0, st>>>(a);
   if (vec == 8) {
     TOMO_ADJ3(8)
   } else {

@@ -11,7 +11,8 @@ f_p = 0.1, point receivers at the transducers).
   inclusions, forward wavefield checkpointed every 200 steps.
 - ``c4``: 64 phantoms, sides drawn uniformly from {128, 192, 256, 384, 512},
   n/4 transducers, 8 n steps, inclusions as in c1; checkpoint every 128
-  steps, 32 shots per pass.
+  steps, 32 shots per pass (or all shots in one pass where the engines
+  share wavefield workspaces).
 
 Ring radius 0.39 n unless stated; transducers fire one at a time; seeds are
 fixed per workload.
@@ -81,13 +82,15 @@ def c4_sides(count: int = 64, seed: int = 4000):
     return [int(C4_SIDES[i]) for i in rng.integers(0, len(C4_SIDES), count)]
 
 
-def c4_problems(count: int = 64, seed: int = 4000, device: int = 0, log=None):
-    """Engines of the uneven batch (data from the FP64 solver)."""
+def c4_problems(count: int = 64, seed: int = 4000, device: int = 0, log=None, passes: int = 32):
+    """Engines of the uneven batch (data from the FP64 solver); `passes`
+    shots per pass (0: all shots in one pass, for engines that share
+    wavefield workspaces)."""
     engines = []
     for i, n in enumerate(c4_sides(count, seed)):
         c = config_unit(n, 8 * n + 2)
         c.checkpoint_every = 128
-        c.shots_per_pass = 32
+        c.shots_per_pass = passes
         ii, jj = transducers_ring(n, n, n // 4, 0.39 * n)
         fo = c1_phantom(n, seed + 1 + i, (c.mask_x0, c.mask_x1, c.mask_y0, c.mask_y1))
         engines.append(engine_for(c, ii, jj, fo, device=device))
