@@ -351,7 +351,8 @@ int tomo_timing_reset(tomo_ctx *ctx);
 /* Shape of the stencil kernels: vec = cells per lane along x (1 scalar, 2 or
  * 4 vectorised; FP32 solver only, nx % vec == 0); rows per lane along y for
  * the forward step (vec 1: 4/8/16; vec 2/4: 4/8/16/32) and the adjoint step
- * (vec 1: 4/8; vec 2/4: 4/8/16/32); prefetch depth of the vectorised kernels
+ * (vec 1: 4/8; vec 2/4: 4/8/16/32, and 64, which only the three-step adjoint
+ * takes: the other adjoint kernels then run 32); prefetch depth of the vectorised kernels
  * (rows whose loads are in flight ahead of the row being computed: 1, 2, 4).
  * 0 in a field picks the default.  Every shape gives the same results up to
  * FP32 rounding.  No reference counterpart (its launch shapes are fixed,

@@ -145,11 +145,12 @@ def test_coverage_with_blocked_adjoint(n, nt, ckpt, steps):
 @pytest.mark.parametrize("n,nt,ckpt,adj,rows,depth", [(512, 160, 7, 2, 0, 1), (256, 301, 49, 2, 0, 1),
                                                        (512, 203, 50, 3, 16, 1), (256, 241, 40, 3, 32, 1),
                                                        (512, 98, 0, 3, 32, 1), (512, 203, 50, 3, 32, 2),
-                                                       (256, 160, 7, 3, 16, 2)])
+                                                       (256, 160, 7, 3, 16, 2), (512, 203, 50, 3, 64, 1),
+                                                       (256, 241, 40, 3, 64, 1), (512, 98, 0, 3, 64, 1)])
 def test_adjoint_steps_and_strip_heights(n, nt, ckpt, adj, rows, depth):
     """The two-step adjoint where three would apply (tomo_set_adjoint_blocking),
-    and the three-step adjoint with 16- and 32-row strips, full history and
-    one or two rows of read-ahead, against one step per launch."""
+    and the three-step adjoint with 16-, 32- and 64-row strips, full history
+    and one or two rows of read-ahead, against one step per launch."""
     print(f"[case] adjoint steps {adj} rows {rows} depth {depth} n={n} nt={nt} ckpt={ckpt}", flush=True)
     e1, c = engine(n, nt, ckpt, 1)
     e2, _ = engine(n, nt, ckpt, 3)

@@ -93,6 +93,7 @@ struct tomo_ctx {
   // blockings do not apply)
   int time_block = 3;
   int adj_block = 3;  // most adjoint steps per launch (tomo_set_adjoint_blocking)
+  int num_cus = 256;  // compute units of the device
   // per-pixel illumination weights of the updates (0 off, 1 coverage)
   int pixw_mode = 0;
   double pixw_eps = 0.05;

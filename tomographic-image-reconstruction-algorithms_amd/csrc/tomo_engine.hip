@@ -54,7 +54,9 @@ StencilShape stencil_shape(const tomo_ctx *c) {
   // flight, adjoint 16 rows with 1 (its deeper queues cost occupancy).
   s.fwd_rows = c->shape_req.fwd_rows ? c->shape_req.fwd_rows
                                      : (s.vec > 1 ? (c->ny >= 512 ? 32 : c->ny >= 256 ? 16 : 8) : 4);
-  s.adj_rows = c->shape_req.adj_rows ? c->shape_req.adj_rows : (s.vec > 1 ? (c->ny >= 256 ? 16 : 8) : 4);
+  // 64-row strips exist only for the three-step adjoint; the other adjoint
+  // kernels take 32 then
+  s.adj_rows = c->shape_req.adj_rows ? std::min(c->shape_req.adj_rows, 32) : (s.vec > 1 ? (c->ny >= 256 ? 16 : 8) : 4);
   // prefetch depth: 2 rows ahead for the one-step forward; 1 for the
   // two-step forward, whose registers already hold two levels (measured at
   // 512^2 x 128 shots: 129.2 us per launch at depth 1, 133.3 at depth 2)
@@ -72,7 +74,7 @@ StencilShape stencil_shape(const tomo_ctx *c) {
 // rows of z(k+1) per strip, its register window does not grow with it);
 // measured at 512^2 x 128 shots: 32 rows 5 % faster than 16
 int adjoint2_rows(const tomo_ctx *c) {
-  if (c->shape_req.adj_rows) return c->shape_req.adj_rows;
+  if (c->shape_req.adj_rows) return std::min(c->shape_req.adj_rows, 32);
   return c->ny >= 512 ? 32 : c->ny >= 256 ? 16 : 8;
 }
 
@@ -80,9 +82,16 @@ int adjoint2_rows(const tomo_ctx *c) {
 // adjoint depth)
 int adjoint3_depth(const tomo_ctx *c) { return c->shape_req.adj_depth == 2 ? 2 : 1; }
 
-// strip height of the three-step adjoint (16 or 32 rows)
-int adjoint3_rows(const tomo_ctx *c) {
-  if (c->shape_req.adj_rows == 16 || c->shape_req.adj_rows == 32) return c->shape_req.adj_rows;
+// strip height of the three-step adjoint (16, 32 or 64 rows) for passes of
+// np shots.  64-row strips halve the strips' halo rows and start-up against
+// 32-row ones, so they are taken when the pass still has a wave for every
+// slot (one wave per SIMD at nx = 512, two at nx = 256); measured at 512^2 x
+// 128 shots: 525.3 against 517.0 Gcell/s per iteration
+int adjoint3_rows(const tomo_ctx *c, int np) {
+  const int r = c->shape_req.adj_rows;
+  if (r == 16 || r == 32 || r == 64) return r;
+  const long long slots = 4LL * c->num_cus * (c->nx == 512 ? 1 : 2);
+  if ((long long)np * ((c->ny + 63) / 64) >= slots) return 64;
   return c->ny >= 512 ? 32 : 16;
 }
 
@@ -92,7 +101,7 @@ bool shape_valid(int vec, int fr, int ar) {
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4) return false;
   if (vec == 1) return (fr == 0 || fr == 4 || fr == 8 || fr == 16) && (ar == 0 || ar == 4 || ar == 8);
   auto ok = [](int r) { return r == 0 || r == 4 || r == 8 || r == 16 || r == 32; };
-  return ok(fr) && ok(ar);
+  return ok(fr) && (ok(ar) || ar == 64);
 }
 
 hipEvent_t take_event(tomo_ctx *c) {
@@ -793,7 +802,7 @@ int gradient_pass(tomo_ctx *c, int p0, int np, int P) {
         t.b.um = segment_level(c, P, m, k - 1);
         t.um2 = segment_level(c, P, m, k - 2);
         t.b.level = k;
-        HIPCHK(c, launch_adjoint3(t, np, adjoint3_rows(c), adjoint3_depth(c), c->stream));
+        HIPCHK(c, launch_adjoint3(t, np, adjoint3_rows(c, np), adjoint3_depth(c), c->stream));
         launches += 3;
         q -= 3;
         continue;
@@ -858,6 +867,7 @@ std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
                          (uintptr_t)sh.fwd_rows, (uintptr_t)sh.adj_rows, (uintptr_t)sh.fwd_depth,
                          (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c), (uintptr_t)three_step(c),
                          (uintptr_t)two_step_adjoint(c), (uintptr_t)three_step_adjoint(c),
+                         (uintptr_t)c->shape_req.adj_rows,
                          (uintptr_t)c->d_recvseg};
   return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
 }
@@ -1030,6 +1040,9 @@ int tomo_create(const tomo_config *cfg, int device, tomo_ctx **out) {
     return code;
   };
   if (hipSetDevice(device) != hipSuccess) return bail(fail(c, TOMO_ERR_HIP, "hipSetDevice failed"));
+  if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      c->num_cus <= 0)
+    return bail(fail(c, TOMO_ERR_HIP, "hipDeviceGetAttribute(multiprocessor count) failed"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(c, TOMO_ERR_HIP, "hipStreamCreate failed"));
   if ((rc = dalloc(c, &c->d_f, c->n2)) || (rc = dalloc(c, &c->d_fo, c->n2)) || (rc = dalloc(c, &c->d_pw, c->n2)) ||
@@ -1839,6 +1852,40 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
         best.shape.adj_rows = bra;
         best.adj_ms = ba / 2;
       }
+      // where three adjoint steps per launch apply, that kernel runs all but
+      // the tails: its strip heights are timed instead
+      c->shape_req = best.shape;
+      if (three_step_adjoint(c)) {
+        Adj3Args t{};
+        t.b = adj2_args(c, 0);
+        t.b.z1 = Z1;
+        t.b.z2 = Z2;
+        t.b.zm = c->d_z + 3 * lvl;
+        t.zm2 = c->d_z + 4 * lvl;
+        t.b.u0 = pass1_level(c, P, 0);
+        t.b.um = pass1_level(c, P, 1);
+        t.um2 = pass1_level(c, P, 2);
+        t.b.level = mid;
+        double b3 = 1e30;
+        int br3 = 0;
+        for (int rows : {16, 32, 64}) {
+          if (rows > c->ny) continue;
+          double t3 = 0.0;
+          rc = time_launches(c, reps, [&]() -> int {
+            HIPCHK(c, launch_adjoint3(t, P, rows, 1, c->stream));
+            return TOMO_OK;
+          }, &t3);
+          if (rc) {
+            c->shape_req = keep;
+            return rc;
+          }
+          if (t3 < b3) { b3 = t3; br3 = rows; }
+        }
+        if (br3) {
+          best.shape.adj_rows = br3;
+          best.adj_ms = b3 / 3;
+        }
+      }
     }
   }
   c->shape_req = keep;
@@ -1878,7 +1925,9 @@ int tomo_get_shape(const tomo_ctx *c, int *out5) {
   const StencilShape s = stencil_shape(c);
   out5[0] = s.vec;
   out5[1] = s.fwd_rows;
-  out5[2] = s.adj_rows;
+  // the three-step adjoint's strips where it runs (for passes of the most
+  // shots a pass holds)
+  out5[2] = three_step_adjoint(c) ? adjoint3_rows(c, pass_shots(c)) : s.adj_rows;
   out5[3] = s.fwd_depth;
   out5[4] = s.adj_depth;
   return TOMO_OK;

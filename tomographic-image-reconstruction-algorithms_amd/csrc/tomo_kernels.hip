@@ -1699,12 +1699,18 @@ __device__ __forceinline__ void adjoint3_strip(const Adj3Args &A3, const float2 
   auto rule1 = [&](int y) { return y >= rs1 && y <= re1; };
   auto own = [&](int y) { return y >= qa && y <= qb; };
   auto lrow = [&](int y) { return y >= qa - 1 && y <= qb + 1; };  // u rows of the Laplacians
-  unsigned long long rflags = 0;                                     // rows rs0 .. re0 holding receivers
+  // rows rs0 .. re0 holding receivers, one bit each (RY + 4 rows: two words
+  // at RY = 64)
+  unsigned long long rflags0 = 0, rflags1 = 0;
 #pragma unroll
   for (int i = 0; i < RY + 4; ++i)
-    if (rs0 + i <= re0 && A.row_has_recv[rs0 + i]) rflags |= 1ull << i;
+    if (rs0 + i <= re0 && A.row_has_recv[rs0 + i]) {
+      if (i < 64) rflags0 |= 1ull << i;
+      else rflags1 |= 1ull << (i - 64);
+    }
   auto desc = [&](int y) {
-    const bool on = rule0(y) && ((rflags >> (y - rs0)) & 1ull);
+    const int i = y - rs0;
+    const bool on = rule0(y) && (((i < 64 ? rflags0 >> i : rflags1 >> (i - 64))) & 1ull);
     return on ? A.recv_seg[y * kBlockX + threadIdx.x] : -1;
   };
   auto resid = [&](int d, const float *__restrict__ r) { return d >= 0 ? r[d / VEC] : 0.f; };
@@ -2704,17 +2710,19 @@ bool adjoint3_supported(int nx, int sponge_w) {
   return sponge_w > 0 && sponge_w <= kSpongeLds && (nx == 256 || nx == 512);
 }
 
+// rows: 16, 32 or 64 (64-row strips run with one row of read-ahead)
 hipError_t launch_adjoint3(Adj3Args a, int nshots, int rows, int depth, hipStream_t st) {
   Adj2Args &b = a.b;
   if (!adjoint3_supported(b.nx, b.sponge_w)) return hipErrorInvalidValue;
   const int vec = b.nx / kBlockX;
-  const int ry = rows == 16 ? 16 : 32;
+  const int ry = rows == 16 ? 16 : rows == 64 ? 64 : 32;
   const int strips = (b.ny + ry - 1) / ry;
   b.blocks_per_shot = (strips + kBlockY - 1) / kBlockY;
   const dim3 grid(b.blocks_per_shot * nshots, 1, 1), block(kBlockX, kBlockY, 1);
 #define TOMO_ADJ3(V)                                                                  \
   if (ry == 16 && depth == 2) adjoint3_kernel<V, 16, 2><<<grid, block, 0, st>>>(a);   \
   else if (ry == 16) adjoint3_kernel<V, 16, 1><<<grid, block, 0, st>>>(a);            \
+  else if (ry == 64) adjoint3_kernel<V, 64, 1><<<grid, block, 0, st>>>(a);            \
   else if (depth == 2) adjoint3_kernel<V, 32, 2><<<grid, block, 0, st>>>(a);          \
   else adjoint3_kernel<V, 32, 1><<<grid, block, 0, st>>>(a);
   if (vec == 8) {
