@@ -155,8 +155,25 @@ int tomo_set_transducers(tomo_ctx *ctx, const int *ii, const int *jj, int ns, in
 
 /* Consecutive source groups of `group_size` transducers; group g fires the
  * filled bounding box of transducers g*G and g*G+G-1 (sirt-mimo.cu:157 and
- * 543-554).  Returns the number of groups (shots) or a negative status. */
+ * 543-554).  Returns the number of shots (the groups, once per phantom) or
+ * a negative status. */
 int tomo_set_groups(tomo_ctx *ctx, int group_size);
+
+/* Stack `count` phantoms of this context's geometry (grid, transducers,
+ * groups, receivers, steps) along the shot axis: phantom p owns shots
+ * [p Ng, (p + 1) Ng) and its own field, truth and coefficient image, so one
+ * stencil launch per level advances every shot of every phantom.  Fields
+ * (tomo_set_field / tomo_get_field / tomo_set_truth) become
+ * float[count][ny][nx], coverage maps float[count][ny][nx], tomo_error
+ * writes one epsilon per phantom, and tomo_update_uniform /
+ * tomo_update_weighted update each phantom from its own shots' gradients
+ * (weights_out: double[count][Ng]).  ART, the intermediate-field average,
+ * the line search and the learned updates take one phantom
+ * (TOMO_ERR_STATE).  Resets the fields to 0 and drops the truth.  No
+ * reference counterpart: the reference runs one phantom per process
+ * (sirt-mimo-norm-mem.cu:429-458 splits one phantom's shots into passes). */
+int tomo_set_phantoms(tomo_ctx *ctx, int count);
+int tomo_num_phantoms(const tomo_ctx *ctx);
 
 /* TOMO_RECV_POINTS only: receiver cells (one_based as above). */
 int tomo_set_receivers(tomo_ctx *ctx, const int *ri, const int *rj, int nr, int one_based);
@@ -309,14 +326,16 @@ int tomo_update_avg_weight(tomo_ctx *ctx, float scale, double *weights_out);
 int tomo_update_group(tomo_ctx *ctx, int g, float scale);
 
 /* epsilon = 100 * ||f - fo|| / ||fo|| over the full grid, FP64
- * (sirt-mimo.cu:342-344, 427-436). */
+ * (sirt-mimo.cu:342-344, 427-436); one value per phantom (eps[count] after
+ * tomo_set_phantoms). */
 int tomo_error(tomo_ctx *ctx, double *eps);
 
 /* ------------------------------------------------------------ iterations */
 
 /* One SIRT iteration over all shots (gradient of every shot in passes of
  * shots_per_pass, then the update): weighted != 0 -> tomo_update_weighted,
- * else tomo_update_uniform(mode).  misfit / eps may be NULL. */
+ * else tomo_update_uniform(mode).  misfit / eps may be NULL (eps: one value
+ * per phantom). */
 int tomo_sirt_iteration(tomo_ctx *ctx, float scale, int weighted, int mode,
                         double *misfit, double *eps);
 
@@ -415,7 +434,12 @@ int tomo_autotune(tomo_ctx *ctx, int force, int *out5, double *ms2);
  * reference's per-step launch sequence (sirt-mimo.cu:326-366) with one
  * launch per pass. */
 int tomo_set_graphs(tomo_ctx *ctx, int on);
-/* replayed-graph time: out3 = {ms, kernel launches covered, cell updates} */
+/* Replayed graphs: out3 = {ms of the replays, replays, cell updates of
+ * replays whose loops were not timed inside the graph}.  A captured pass
+ * records its stencil loops' event pairs as event-record nodes of the graph,
+ * so on replay they are timed and counted in tomo_timing like eager loops;
+ * where the runtime refused such nodes at capture, the replay's cell updates
+ * are counted here instead. */
 int tomo_timing_graphs(tomo_ctx *ctx, double *out3);
 /* out5 = {vec, fwd_rows, adj_rows, fwd_depth, adj_depth} in effect */
 int tomo_get_shape(const tomo_ctx *ctx, int *out5);

@@ -57,7 +57,44 @@ def test_graph_replays_match_eager(n, ckpt):
             graph.misfit()
             graph.forward()
         t = graph.timing_graphs()
-        assert t["launches"] > 0 and t["ms"] > 0
+        assert t["replays"] > 0 and t["ms"] > 0
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_graph_replays_time_their_stencil_loops():
+    """The replayed graph records its loops' event pairs: the forward and
+    adjoint launch and cell counts equal the eager run's, and their times
+    fall inside the replays' own time."""
+    print("[case] graph loop timings", flush=True)
+    c = unit_config(128, 600)
+    c.checkpoint_every = 100
+    ii, jj, _ = unit_geometry(c, 16)
+    fo = unit_phantom(c)
+    with tomo.Engine(c, device=0) as g:
+        g.set_transducers(ii, jj, one_based=False)
+        g.set_receivers(ii, jj, one_based=False)
+        g.set_groups(1)
+        g.set_field(fo)
+        meas = g.forward(fp64=True)
+    eager = make(c, ii, jj, meas, False)
+    graph = make(c, ii, jj, meas, True)
+    try:
+        for e in (eager, graph):
+            e.set_field((0.5 * fo).astype(np.float32))
+            e.gradient()  # the graph is captured here
+            e.timing_reset()
+            for _ in range(3):
+                e.gradient()
+        te, tg, tr = eager.timing(), graph.timing(), graph.timing_graphs()
+        for k in ("fwd_launches", "adj_launches", "fwd_cells", "adj_cells"):
+            assert tg[k] == te[k], (k, tg[k], te[k])
+        assert tr["replays"] == 3 and tr["untimed_cells"] == 0
+        inner = tg["fwd_ms"] + tg["adj_ms"]
+        print(f"[case] graph replays {tr['ms']:.3f} ms, stencil loops inside {inner:.3f} ms, "
+              f"eager loops {te['fwd_ms'] + te['adj_ms']:.3f} ms", flush=True)
+        assert 0.3 * tr["ms"] < inner <= 1.001 * tr["ms"]
     finally:
         eager.close()
         graph.close()

@@ -89,6 +89,8 @@ SIGNATURES = {
     "tomo_num_shots": (_int, [_vp]),
     "tomo_num_receivers": (_int, [_vp]),
     "tomo_get_receivers": (_int, [_vp, _i32p, _i32p]),
+    "tomo_set_phantoms": (_int, [_vp, _int]),
+    "tomo_num_phantoms": (_int, [_vp]),
     "tomo_set_field": (_int, [_vp, _f32p]),
     "tomo_get_field": (_int, [_vp, _f32p]),
     "tomo_set_truth": (_int, [_vp, _f32p]),
@@ -117,7 +119,7 @@ SIGNATURES = {
     "tomo_update_weighted_noalloc": (_int, [_vp, _vp, _int, _int, _int, _i32p, ctypes.c_double, _vp, _vp, _vp]),
     "tomo_update_avg_weight": (_int, [_vp, ctypes.c_float, _vp]),
     "tomo_update_group": (_int, [_vp, _int, ctypes.c_float]),
-    "tomo_error": (_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "tomo_error": (_int, [_vp, _f64p]),
     "tomo_sirt_iteration": (_int, [_vp, ctypes.c_float, _int, _int, _vp, _vp]),
     "tomo_art_iteration": (_int, [_vp, ctypes.c_float, _vp, _vp]),
     "tomo_schedule": (ctypes.c_double, [_int, _int, _f64p]),
@@ -288,6 +290,7 @@ class Engine:
         self._ctx = _vp()
         _check(lib().tomo_create(ctypes.byref(self.cfg), int(device), ctypes.byref(self._ctx)))
         self.n2 = cfg.nx * cfg.ny
+        self.phantoms = 1
 
     def close(self):
         if self._ctx:
@@ -316,6 +319,13 @@ class Engine:
     def set_groups(self, group_size: int) -> int:
         return self._c(lib().tomo_set_groups(self._ctx, int(group_size)))
 
+    def set_phantoms(self, count: int):
+        """Stack `count` phantoms of this geometry along the shot axis
+        (fields, truths and coverage maps become [count][ny][nx]; error()
+        and sirt_iteration() return one epsilon per phantom)."""
+        self._c(lib().tomo_set_phantoms(self._ctx, int(count)))
+        self.phantoms = int(count)
+
     def set_receivers(self, ri, rj, one_based=False):
         self._c(lib().tomo_set_receivers(self._ctx, _i32(ri), _i32(rj), len(ri), int(one_based)))
 
@@ -339,7 +349,7 @@ class Engine:
         self._c(lib().tomo_set_field(self._ctx, _f32(f).ravel()))
 
     def get_field(self):
-        f = np.zeros(self.n2, np.float32)
+        f = np.zeros(self.n2 * self.phantoms, np.float32)
         self._c(lib().tomo_get_field(self._ctx, f))
         return f
 
@@ -392,7 +402,7 @@ class Engine:
         self._c(lib().tomo_enable_coverage(self._ctx, int(bool(on))))
 
     def get_coverage(self):
-        out = np.zeros(self.n2, np.float32)
+        out = np.zeros(self.n2 * self.phantoms, np.float32)
         self._c(lib().tomo_get_coverage(self._ctx, out))
         return out
 
@@ -427,18 +437,18 @@ class Engine:
     def update_group(self, g, scale):
         self._c(lib().tomo_update_group(self._ctx, int(g), float(scale)))
 
-    def error(self) -> float:
-        e = ctypes.c_double(0.0)
-        self._c(lib().tomo_error(self._ctx, ctypes.byref(e)))
-        return e.value
+    def error(self):
+        """epsilon in %; an array of one value per phantom when phantoms are stacked."""
+        e = np.zeros(self.phantoms, np.float64)
+        self._c(lib().tomo_error(self._ctx, e))
+        return float(e[0]) if self.phantoms == 1 else e
 
     def sirt_iteration(self, scale, weighted=False, mode=UPDATE_MEAN):
         mis = ctypes.c_double(0.0)
-        eps = ctypes.c_double(0.0)
+        eps = np.zeros(self.phantoms, np.float64)
         self._c(lib().tomo_sirt_iteration(self._ctx, float(scale), int(bool(weighted)), int(mode),
-                                          ctypes.cast(ctypes.byref(mis), _vp),
-                                          ctypes.cast(ctypes.byref(eps), _vp)))
-        return mis.value, eps.value
+                                          ctypes.cast(ctypes.byref(mis), _vp), _ptr(eps)))
+        return mis.value, (float(eps[0]) if self.phantoms == 1 else eps)
 
     def art_iteration(self, scale):
         mis = ctypes.c_double(0.0)
@@ -460,7 +470,7 @@ class Engine:
     def timing_graphs(self) -> dict:
         out = np.zeros(3, np.float64)
         self._c(lib().tomo_timing_graphs(self._ctx, out))
-        return {"ms": out[0], "launches": int(out[1]), "cells": out[2]}
+        return {"ms": out[0], "replays": int(out[1]), "untimed_cells": out[2]}
 
     def set_time_blocking(self, steps: int):
         """Forward steps per launch in the forward passes (1, 2 or 3; default 3)."""

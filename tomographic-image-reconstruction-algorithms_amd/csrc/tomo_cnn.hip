@@ -933,6 +933,7 @@ int tomo_cnn_add_sample(tomo_cnn *n, tomo_ctx *c, float scale) {
   if (!n || !c) return TOMO_ERR_ARG;
   int rc = check_ctx(c);
   if (rc) return rc;
+  if (c->nph != 1) return cfail(n, TOMO_ERR_STATE, "tomo_cnn_add_sample takes a context of one phantom");
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return cfail(n, rc, c->err);
   if (!c->have_truth) return cfail(n, TOMO_ERR_STATE, "the context has no ground truth");
   if (c->nx % 32 != 0) return cfail(n, TOMO_ERR_ARG, "grid width must be a multiple of 32");
@@ -1038,6 +1039,7 @@ int tomo_learned_update(tomo_ctx *c, tomo_cnn *n, float alpha, float scale) {
   if (!n || !c) return TOMO_ERR_ARG;
   int rc = check_ctx(c);
   if (rc) return rc;
+  if (c->nph != 1) return fail(c, TOMO_ERR_STATE, "tomo_learned_update takes a context of one phantom");
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
   if (c->nx % 32 != 0) return fail(c, TOMO_ERR_ARG, "grid width must be a multiple of 32");
   if ((rc = engine_input(n, c, scale, nullptr))) return fail(c, rc, n->err);
@@ -1070,6 +1072,7 @@ int tomo_learned_iteration(tomo_ctx *c, tomo_cnn *n, float alpha, float scale, i
   if (!n || !c) return TOMO_ERR_ARG;
   int rc = check_ctx(c);
   if (rc) return rc;
+  if (c->nph != 1) return fail(c, TOMO_ERR_STATE, "tomo_learned_iteration takes a context of one phantom");
   if ((rc = need_shots(c))) return rc;
   if (c->nx % 32 != 0) return fail(c, TOMO_ERR_ARG, "grid width must be a multiple of 32");
   if (!(alpha > 0.f)) return fail(c, TOMO_ERR_ARG, "alpha must be positive");

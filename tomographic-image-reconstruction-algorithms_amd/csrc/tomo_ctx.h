@@ -16,19 +16,24 @@
 constexpr int kAdjRing = 5;
 
 struct TimedSpan {
-  hipEvent_t a, b;
-  int kind;          // 0 forward, 1 adjoint, 2 graph replay
-  double launches;
-  double cells;
+  hipEvent_t a = nullptr, b = nullptr;
+  int kind = 0;          // 0 forward, 1 adjoint, 2 graph replay
+  double launches = 0;
+  double cells = 0;
+  bool owned = false;    // events belong to a captured graph (not recycled)
 };
 
 // One captured gradient pass (shots [p0, p0 + np) in passes of P) and the
-// pointers and parameters its launches baked in.
+// pointers and parameters its launches baked in.  spans: the stencil loops'
+// event pairs, recorded by event-record nodes inside the graph on every
+// replay (empty when the runtime refused such nodes at capture; launches and
+// cells then count for the replay as a whole).
 struct GraphEntry {
   int p0 = 0, np = 0, P = 0;
   std::vector<uintptr_t> sig;
   hipGraphExec_t exec = nullptr;
   double launches[2] = {0, 0}, cells[2] = {0, 0};
+  std::vector<TimedSpan> spans;
 };
 
 // Wavefield storage shared by contexts that are never used at the same time
@@ -53,6 +58,10 @@ struct tomo_ctx {
   std::vector<int4> bbox;   // per shot
   std::vector<int> rcell, ranchor;
   int nshots = 0, nr = 0;
+  // phantoms stacked along the shot axis (tomo_set_phantoms): shots
+  // [p groups, (p + 1) groups) belong to phantom p, whose field, truth and
+  // coefficient images are n2 elements apart; nshots = nph * groups
+  int nph = 1, groups = 0;
   // persistent device buffers
   float *d_f = nullptr, *d_fo = nullptr, *d_pw = nullptr;
   double *d_pw64 = nullptr;
@@ -102,6 +111,8 @@ struct tomo_ctx {
   bool use_graphs = false;
   bool capturing = false;
   double cap_launches[2] = {0, 0}, cap_cells[2] = {0, 0};
+  std::vector<TimedSpan> cap_spans;  // event pairs recorded into the graph being captured
+  bool cap_events_ok = true;         // every event-record node of this capture was accepted
   std::vector<GraphEntry> graphs;
 };
 

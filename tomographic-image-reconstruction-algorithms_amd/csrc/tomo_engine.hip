@@ -115,12 +115,27 @@ hipEvent_t take_event(tomo_ctx *c) {
   return e;
 }
 
+// Inside a graph capture a span's events become event-record nodes of the
+// graph (hipEventRecordExternal): every replay records them, so the stencil
+// loops of a replayed pass are timed like eager ones.  The events then belong
+// to the graph entry.
 void span_begin(tomo_ctx *c, TimedSpan &s, int kind) {
   s.kind = kind;
   s.launches = 0;
   s.cells = 0;
-  if (c->capturing) {  // no events inside a captured graph: counts only
-    s.a = s.b = nullptr;
+  s.owned = false;
+  s.a = s.b = nullptr;
+  if (c->capturing) {
+    if (!c->cap_events_ok) return;
+    if (hipEventCreate(&s.a) != hipSuccess || hipEventCreate(&s.b) != hipSuccess ||
+        hipEventRecordWithFlags(s.a, c->stream, hipEventRecordExternal) != hipSuccess) {
+      (void)hipGetLastError();
+      c->cap_events_ok = false;
+      if (s.a) hipEventDestroy(s.a);
+      if (s.b) hipEventDestroy(s.b);
+      s.a = s.b = nullptr;
+    }
+    s.owned = true;
     return;
   }
   s.a = take_event(c);
@@ -132,15 +147,35 @@ void span_end(tomo_ctx *c, TimedSpan &s) {
   if (c->capturing) {
     c->cap_launches[s.kind] += s.launches;
     c->cap_cells[s.kind] += s.cells;
+    if (s.a && s.b) {
+      if (hipEventRecordWithFlags(s.b, c->stream, hipEventRecordExternal) == hipSuccess) {
+        c->cap_spans.push_back(s);
+        return;
+      }
+      (void)hipGetLastError();
+      c->cap_events_ok = false;
+    }
+    if (s.a) hipEventDestroy(s.a);
+    if (s.b) hipEventDestroy(s.b);
     return;
   }
   if (s.b) hipEventRecord(s.b, c->stream);
   c->spans.push_back(s);
 }
 
+void destroy_spans(std::vector<TimedSpan> &v) {
+  for (auto &s : v) {
+    if (s.a) hipEventDestroy(s.a);
+    if (s.b) hipEventDestroy(s.b);
+  }
+  v.clear();
+}
+
 void graphs_clear(tomo_ctx *c) {
-  for (auto &g : c->graphs)
+  for (auto &g : c->graphs) {
     if (g.exec) hipGraphExecDestroy(g.exec);
+    destroy_spans(g.spans);
+  }
   c->graphs.clear();
 }
 
@@ -153,6 +188,7 @@ void spans_collect(tomo_ctx *c) {
       c->t_launch[s.kind] += s.launches;
       c->t_cells[s.kind] += s.cells;
     }
+    if (s.owned) continue;
     if (s.a) c->ev_free.push_back(s.a);
     if (s.b) c->ev_free.push_back(s.b);
   }
@@ -165,8 +201,9 @@ namespace tomo_internal {
 
 int upload_coefficients(tomo_ctx *c) {
   const double c0dt_h = c->cfg.c0 * c->cfg.dt / c->cfg.h;
-  HIPCHK(c, launch_coefficients<float>(c->d_f, c->d_pw, c->nx, c->ny, c0dt_h, c->cfg.sponge_width, c->d_spa,
-                                       c->stream));
+  for (int p = 0; p < c->nph; ++p)
+    HIPCHK(c, launch_coefficients<float>(c->d_f + (size_t)p * c->n2, c->d_pw + (size_t)p * c->n2, c->nx, c->ny,
+                                         c0dt_h, c->cfg.sponge_width, c->d_spa, c->stream));
   c->pw64_valid = false;
   return TOMO_OK;
 }
@@ -179,12 +216,13 @@ namespace {
 int ensure_pw64(tomo_ctx *c) {
   if (c->pw64_valid) return TOMO_OK;
   if (!c->d_pw64) {
-    int rc = dalloc(c, &c->d_pw64, (size_t)c->n2);
+    int rc = dalloc(c, &c->d_pw64, (size_t)c->nph * c->n2);
     if (rc) return rc;
   }
   const double c0dt_h = c->cfg.c0 * c->cfg.dt / c->cfg.h;
-  HIPCHK(c, launch_coefficients<double>(c->d_f, c->d_pw64, c->nx, c->ny, c0dt_h, c->cfg.sponge_width,
-                                        c->d_spa64, c->stream));
+  for (int p = 0; p < c->nph; ++p)
+    HIPCHK(c, launch_coefficients<double>(c->d_f + (size_t)p * c->n2, c->d_pw64 + (size_t)p * c->n2, c->nx, c->ny,
+                                          c0dt_h, c->cfg.sponge_width, c->d_spa64, c->stream));
   c->pw64_valid = true;
   return TOMO_OK;
 }
@@ -418,6 +456,8 @@ FwdArgs<T> fwd_args(tomo_ctx *c, int shot0, const T *pw, const T *spa, const T *
   a.sponge_w = c->cfg.sponge_width;
   a.sp_a = spa;
   a.sp_ab = spab;
+  a.spp = c->nph > 1 ? c->groups : 0;
+  a.pstride = c->n2;
   return a;
 }
 
@@ -466,6 +506,8 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
   b.sstride = a.sstride;
   b.pw = a.pw;
   b.f = a.f;
+  b.spp = a.spp;
+  b.pstride = a.pstride;
   b.bbox = a.bbox;
   b.nx = c->nx;
   b.ny = c->ny;
@@ -497,6 +539,8 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
     e.up2 = up2;
     e.sstride = a.sstride;
     e.f = a.f;
+    e.spp = a.spp;
+    e.pstride = a.pstride;
     e.ring = c->d_ring + 2 * c->nx;
     e.nring = c->nr;
     e.nx = c->nx;
@@ -531,6 +575,8 @@ int forward_step3(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
   b.sstride = a.sstride;
   b.pw = a.pw;
   b.f = a.f;
+  b.spp = a.spp;
+  b.pstride = a.pstride;
   b.bbox = a.bbox;
   b.nx = c->nx;
   b.ny = c->ny;
@@ -563,6 +609,8 @@ int forward_step3(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
     Fwd2EdgeArgs e{};
     e.sstride = a.sstride;
     e.f = a.f;
+    e.spp = a.spp;
+    e.pstride = a.pstride;
     e.ring = c->d_ring + 2 * c->nx;
     e.nring = c->nr;
     e.nx = c->nx;
@@ -668,6 +716,8 @@ AdjArgs adj_args(tomo_ctx *c, int shot0) {
   a.sp_a = c->d_spa;
   a.sp_ab = c->d_spab;
   a.row_has_recv = c->d_rowflag;
+  a.spp = c->nph > 1 ? c->groups : 0;
+  a.pstride = c->n2;
   return a;
 }
 
@@ -689,6 +739,8 @@ Adj2Args adj2_args(tomo_ctx *c, int shot0) {
   a.sponge_w = c->cfg.sponge_width;
   a.sp_a = c->d_spa;
   a.sp_ab = c->d_spab;
+  a.spp = c->nph > 1 ? c->groups : 0;
+  a.pstride = c->n2;
   return a;
 }
 
@@ -868,7 +920,7 @@ std::vector<uintptr_t> graph_signature(const tomo_ctx *c) {
                          (uintptr_t)sh.adj_depth, (uintptr_t)c->cfg.checkpoint_every, (uintptr_t)two_step(c), (uintptr_t)three_step(c),
                          (uintptr_t)two_step_adjoint(c), (uintptr_t)three_step_adjoint(c),
                          (uintptr_t)c->shape_req.adj_rows,
-                         (uintptr_t)c->d_recvseg};
+                         (uintptr_t)c->d_recvseg, (uintptr_t)c->nph};
   return std::vector<uintptr_t>(v, v + sizeof(v) / sizeof(v[0]));
 }
 
@@ -882,24 +934,31 @@ int gradient_pass_graph(tomo_ctx *c, int p0, int np, int P) {
   if (ge && ge->sig != sig) {
     hipGraphExecDestroy(ge->exec);
     ge->exec = nullptr;
+    destroy_spans(ge->spans);
   }
   if (!ge || !ge->exec) {
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     c->cap_launches[0] = c->cap_launches[1] = c->cap_cells[0] = c->cap_cells[1] = 0;
+    c->cap_spans.clear();
+    c->cap_events_ok = true;
     int rc = gradient_pass(c, p0, np, P);
     c->capturing = false;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc || e != hipSuccess) {
       if (graph) hipGraphDestroy(graph);
+      destroy_spans(c->cap_spans);
       (void)hipGetLastError();
       return rc ? rc : fail(c, TOMO_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     }
     hipGraphExec_t exec = nullptr;
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
-    if (ei != hipSuccess) return fail(c, TOMO_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    if (ei != hipSuccess) {
+      destroy_spans(c->cap_spans);
+      return fail(c, TOMO_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
     if (!ge) {
       c->graphs.emplace_back();
       ge = &c->graphs.back();
@@ -913,13 +972,29 @@ int gradient_pass_graph(tomo_ctx *c, int p0, int np, int P) {
       ge->launches[k] = c->cap_launches[k];
       ge->cells[k] = c->cap_cells[k];
     }
+    // the loops' timings need every span of the pass; a partial set is dropped
+    ge->spans.swap(c->cap_spans);
+    if (!c->cap_events_ok) destroy_spans(ge->spans);
+    c->cap_spans.clear();
+  }
+  // the graph's own event pairs hold one replay's times: the spans of an
+  // earlier replay that nobody has read yet are read before they are overwritten
+  if (!ge->spans.empty()) {
+    bool pending = false;
+    for (const auto &s : c->spans)
+      if (s.owned && s.a == ge->spans.front().a) pending = true;
+    if (pending) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      spans_collect(c);
+    }
   }
   TimedSpan s;
   span_begin(c, s, 2);
   HIPCHK(c, hipGraphLaunch(ge->exec, c->stream));
-  s.launches = ge->launches[0] + ge->launches[1];
-  s.cells = ge->cells[0] + ge->cells[1];
+  s.launches = 1;  // replays; the launches and cells are the loops' own spans
+  if (ge->spans.empty()) s.cells = ge->cells[0] + ge->cells[1];  // no inner timings
   span_end(c, s);
+  for (const auto &g : ge->spans) c->spans.push_back(g);
   return TOMO_OK;
 }
 
@@ -1001,6 +1076,30 @@ int wavefields(tomo_ctx *c, int P) { return ensure_wavefields(c, P, true); }
 int shots_per_pass(const tomo_ctx *c) { return pass_shots(c); }
 
 }  // namespace tomo_internal
+
+namespace {
+
+// the shots of every phantom: the group boxes repeated per phantom, the
+// per-shot buffers sized for nph * groups shots
+int stack_shots(tomo_ctx *c) {
+  const int ng = c->groups;
+  std::vector<int4> all((size_t)ng * c->nph);
+  for (int p = 0; p < c->nph; ++p) std::copy(c->bbox.begin(), c->bbox.end(), all.begin() + (size_t)p * ng);
+  int rc;
+  if ((rc = dalloc(c, &c->d_bbox, all.size()))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_bbox, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
+  c->nshots = (int)all.size();
+  if ((rc = alloc_shot_buffers(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int one_phantom(tomo_ctx *c, const char *what) {
+  if (c->nph == 1) return TOMO_OK;
+  return fail(c, TOMO_ERR_STATE, std::string(what) + " takes a context of one phantom (tomo_set_phantoms)");
+}
+
+}  // namespace
 
 // ====================================================================== ABI
 extern "C" {
@@ -1092,6 +1191,7 @@ void tomo_destroy(tomo_ctx *c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (auto &s : c->spans) {
+    if (s.owned) continue;  // destroyed with their graph
     if (s.a) hipEventDestroy(s.a);
     if (s.b) hipEventDestroy(s.b);
   }
@@ -1148,13 +1248,32 @@ int tomo_set_groups(tomo_ctx *c, int G) {
     c->bbox[g] = make_int4(std::min(c->tx[p], c->tx[q]), std::max(c->tx[p], c->tx[q]), std::min(c->ty[p], c->ty[q]),
                            std::max(c->ty[p], c->ty[q]));
   }
-  if ((rc = dalloc(c, &c->d_bbox, ng))) return rc;
-  HIPCHK(c, hipMemcpy(c->d_bbox, c->bbox.data(), sizeof(int4) * ng, hipMemcpyHostToDevice));
-  c->nshots = ng;
-  if ((rc = alloc_shot_buffers(c))) return rc;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return ng;
+  c->groups = ng;
+  if ((rc = stack_shots(c))) return rc;
+  return c->nshots;
 }
+
+int tomo_set_phantoms(tomo_ctx *c, int count) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (count < 1 || count > 65536) return fail(c, TOMO_ERR_ARG, "phantom count out of range");
+  if ((double)count * c->n2 > 4e9) return fail(c, TOMO_ERR_ARG, "phantom images exceed 2^32 cells");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  graphs_clear(c);
+  c->nph = count;
+  const size_t n = (size_t)count * c->n2;
+  if ((rc = dalloc(c, &c->d_f, n)) || (rc = dalloc(c, &c->d_fo, n)) || (rc = dalloc(c, &c->d_pw, n))) return rc;
+  dfree(c->d_pw64);
+  HIPCHK(c, launch_fill_f32(c->d_f, n, 0.f, c->stream));
+  HIPCHK(c, launch_fill_f32(c->d_fo, n, 0.f, c->stream));
+  c->have_truth = false;
+  if ((rc = upload_coefficients(c))) return rc;
+  if (c->groups > 0 && !c->bbox.empty() && (rc = stack_shots(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TOMO_OK;
+}
+
+int tomo_num_phantoms(const tomo_ctx *c) { return c ? c->nph : TOMO_ERR_ARG; }
 
 int tomo_set_receivers(tomo_ctx *c, const int *ri, const int *rj, int nr, int one_based) {
   int rc = check_ctx(c);
@@ -1196,7 +1315,7 @@ int tomo_set_field(tomo_ctx *c, const float *f) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!f) return fail(c, TOMO_ERR_ARG, "null field");
-  HIPCHK(c, hipMemcpyAsync(c->d_f, f, sizeof(float) * c->n2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_f, f, sizeof(float) * c->n2 * c->nph, hipMemcpyHostToDevice, c->stream));
   if ((rc = upload_coefficients(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TOMO_OK;
@@ -1206,7 +1325,7 @@ int tomo_get_field(tomo_ctx *c, float *f) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!f) return fail(c, TOMO_ERR_ARG, "null field");
-  HIPCHK(c, hipMemcpyAsync(f, c->d_f, sizeof(float) * c->n2, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(f, c->d_f, sizeof(float) * c->n2 * c->nph, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TOMO_OK;
 }
@@ -1215,7 +1334,7 @@ int tomo_set_truth(tomo_ctx *c, const float *fo) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!fo) return fail(c, TOMO_ERR_ARG, "null truth");
-  HIPCHK(c, hipMemcpyAsync(c->d_fo, fo, sizeof(float) * c->n2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_fo, fo, sizeof(float) * c->n2 * c->nph, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_truth = true;
   return TOMO_OK;
@@ -1343,11 +1462,13 @@ int tomo_get_coverage(tomo_ctx *c, float *out) {
   std::vector<float> all((size_t)c->nshots * c->n2);
   HIPCHK(c, hipMemcpyAsync(all.data(), c->d_cov, sizeof(float) * all.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (long long id = 0; id < c->n2; ++id) {
-    double s = 0.0;
-    for (int g = 0; g < c->nshots; ++g) s += all[(size_t)g * c->n2 + id];
-    out[id] = (float)s;
-  }
+  const int G = c->nshots / c->nph;
+  for (int p = 0; p < c->nph; ++p)
+    for (long long id = 0; id < c->n2; ++id) {
+      double s = 0.0;
+      for (int g = p * G; g < (p + 1) * G; ++g) s += all[(size_t)g * c->n2 + id];
+      out[(size_t)p * c->n2 + id] = (float)s;
+    }
   return TOMO_OK;
 }
 
@@ -1356,11 +1477,15 @@ int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
-  const double div = mode == TOMO_UPDATE_MEAN ? (double)c->nshots : 1.0;
-  const float *pw = nullptr;
-  if ((rc = pixel_weights(c, 0, c->nshots, &pw))) return rc;
-  HIPCHK(c, launch_update_uniform(c->d_f, c->d_grad, c->nshots, c->nx, c->ny, ctx_mask(c), (double)scale, div,
-                                  c->stream, pw));
+  // each phantom from its own shots' gradients
+  const int G = c->nshots / c->nph;
+  const double div = mode == TOMO_UPDATE_MEAN ? (double)G : 1.0;
+  for (int p = 0; p < c->nph; ++p) {
+    const float *pw = nullptr;
+    if ((rc = pixel_weights(c, p * G, G, &pw))) return rc;
+    HIPCHK(c, launch_update_uniform(c->d_f + (size_t)p * c->n2, c->d_grad + (size_t)p * G * c->n2, G, c->nx, c->ny,
+                                    ctx_mask(c), (double)scale, div, c->stream, pw));
+  }
   if ((rc = upload_coefficients(c))) return rc;
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1371,30 +1496,33 @@ int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
-  const int M = c->nshots;
+  // each phantom from its own M shots, one after another on the stream
+  // (they share the scratch)
+  const int M = c->nshots / c->nph;
   const int pb = weighted_part_blocks((int)c->n2);
   const size_t need = (size_t)c->n2 + (size_t)M * pb + (size_t)M + 1;
-  if ((rc = ensure_scratch(c, need))) return rc;
+  if ((rc = ensure_scratch(c, need + 2))) return rc;
+  // the weights' own scratch: a separate slot past the weighted-update scratch
   double *mean = c->d_scratch, *part = mean + c->n2, *w = part + (size_t)M * pb;
-  const float *pw = nullptr;
   if (c->pixw_mode) {
-    // the weights' own scratch: a separate slot past the weighted-update scratch
-    if ((rc = ensure_scratch(c, need + 2))) return rc;
-    mean = c->d_scratch;
-    part = mean + c->n2;
-    w = part + (size_t)M * pb;
     if (!c->coverage || !c->d_cov) return fail(c, TOMO_ERR_STATE, "coverage weights need tomo_enable_coverage");
     if (!c->d_pixw && (rc = dalloc(c, &c->d_pixw, (size_t)c->n2))) return rc;
-    HIPCHK(c, launch_pixel_weights(c->d_cov, M, c->nx, c->ny, ctx_mask(c), c->pixw_eps, c->d_scratch + need,
-                                   c->d_pixw, c->stream));
-    pw = c->d_pixw;
   }
-  HIPCHK(c, launch_weighted(c->d_f, c->d_grad, M, c->nx, c->ny, ctx_mask(c), (double)scale, mean, part, pb, w,
-                            c->stream, pw));
+  for (int p = 0; p < c->nph; ++p) {
+    const float *pw = nullptr;
+    if (c->pixw_mode) {
+      HIPCHK(c, launch_pixel_weights(c->d_cov + (size_t)p * M * c->n2, M, c->nx, c->ny, ctx_mask(c), c->pixw_eps,
+                                     c->d_scratch + need, c->d_pixw, c->stream));
+      pw = c->d_pixw;
+    }
+    HIPCHK(c, launch_weighted(c->d_f + (size_t)p * c->n2, c->d_grad + (size_t)p * M * c->n2, M, c->nx, c->ny,
+                              ctx_mask(c), (double)scale, mean, part, pb, w, c->stream, pw));
+    if (weights_out)
+      HIPCHK(c, hipMemcpyAsync(weights_out + (size_t)p * M, w, sizeof(double) * M, hipMemcpyDeviceToHost,
+                               c->stream));
+  }
   if ((rc = upload_coefficients(c))) return rc;
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
-  if (weights_out)
-    HIPCHK(c, hipMemcpyAsync(weights_out, w, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TOMO_OK;
 }
@@ -1402,6 +1530,7 @@ int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
 int tomo_update_avg_weight(tomo_ctx *c, float scale, double *weights_out) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if ((rc = one_phantom(c, "tomo_update_avg_weight"))) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
   const int M = c->nshots;
   const int pb = weighted_part_blocks((int)c->n2);
@@ -1421,6 +1550,7 @@ int tomo_update_avg_weight(tomo_ctx *c, float scale, double *weights_out) {
 int tomo_update_group(tomo_ctx *c, int g, float scale) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if ((rc = one_phantom(c, "tomo_update_group"))) return rc;
   if ((rc = need_shots(c))) return rc;
   if (g < 0 || g >= c->nshots) return fail(c, TOMO_ERR_ARG, "bad shot index");
   if (!c->grad_valid[g]) return fail(c, TOMO_ERR_STATE, "gradient of this shot not computed");
@@ -1440,12 +1570,15 @@ int tomo_error(tomo_ctx *c, double *eps) {
   if (rc) return rc;
   if (!eps) return fail(c, TOMO_ERR_ARG, "null output");
   if (!c->have_truth) return fail(c, TOMO_ERR_STATE, "no ground truth: call tomo_set_truth first");
-  if ((rc = ensure_scratch(c, 2))) return rc;
-  HIPCHK(c, launch_error(c->d_f, c->d_fo, (int)c->n2, c->d_scratch, c->stream));
-  double s[2];
-  HIPCHK(c, hipMemcpyAsync(s, c->d_scratch, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  // one value per phantom
+  if ((rc = ensure_scratch(c, 2 * (size_t)c->nph))) return rc;
+  for (int p = 0; p < c->nph; ++p)
+    HIPCHK(c, launch_error(c->d_f + (size_t)p * c->n2, c->d_fo + (size_t)p * c->n2, (int)c->n2,
+                           c->d_scratch + 2 * p, c->stream));
+  std::vector<double> s(2 * (size_t)c->nph);
+  HIPCHK(c, hipMemcpyAsync(s.data(), c->d_scratch, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  *eps = 100.0 * std::sqrt(s[0]) / std::sqrt(s[1]);
+  for (int p = 0; p < c->nph; ++p) eps[p] = 100.0 * std::sqrt(s[2 * p]) / std::sqrt(s[2 * p + 1]);
   return TOMO_OK;
 }
 
@@ -1461,6 +1594,7 @@ int tomo_sirt_iteration(tomo_ctx *c, float scale, int weighted, int mode, double
 int tomo_art_iteration(tomo_ctx *c, float scale, double *misfit, double *eps) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if ((rc = one_phantom(c, "tomo_art_iteration"))) return rc;
   if ((rc = need_shots(c))) return rc;
   double total = 0.0;
   for (int g = 0; g < c->nshots; ++g) {
@@ -1509,6 +1643,7 @@ int tomo_line_search(tomo_ctx *c, int weighted, double s_lo, double s_step, int 
                      double *best_eps) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if ((rc = one_phantom(c, "tomo_line_search"))) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
   if (!c->have_truth) return fail(c, TOMO_ERR_STATE, "no ground truth: call tomo_set_truth first");
   if (count <= 0 || !(s_step > 0.0) || !best_scale) return fail(c, TOMO_ERR_ARG, "bad scale grid");

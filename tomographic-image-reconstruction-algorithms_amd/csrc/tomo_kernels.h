@@ -48,6 +48,10 @@ struct FwdArgs {
   const T *sp_a, *sp_ab;
   const int2 *ring;     // cells of the edge/receiver pass {flat cell, receiver or -1}
   int nring;
+  // stacked phantoms: shot gs reads the coefficient (and field) image of
+  // phantom gs / spp, pstride elements after phantom 0's; spp = 0: one image
+  int spp;
+  long long pstride;
 };
 
 struct AdjArgs {
@@ -73,6 +77,10 @@ struct AdjArgs {
   const float *sp_a, *sp_ab;
   const int2 *ring;     // cells of the edge/receiver pass
   int nring;
+  // stacked phantoms: shot gs reads the coefficient (and field) image of
+  // phantom gs / spp, pstride elements after phantom 0's; spp = 0: one image
+  int spp;
+  long long pstride;
 };
 
 // Kernel parameter of the stencil kernels: one problem (a), or a batch of
@@ -138,6 +146,10 @@ struct Fwd2Args {
   int nt, nr;
   int level1;            // k+1
   int rec1, rec2;        // record levels k+1, k+2
+  // stacked phantoms: shot gs reads the coefficient (and field) image of
+  // phantom gs / spp, pstride elements after phantom 0's; spp = 0: one image
+  int spp;
+  long long pstride;
 };
 bool forward2_supported(int nx, int sponge_w);
 // depth: rows whose loads are in flight ahead of the row being formed (1, 2)
@@ -177,6 +189,10 @@ struct Fwd2EdgeArgs {
   const float *meas;      // measured traces (null: record u itself)
   float *rec;
   float c0dt_h;
+  // stacked phantoms: shot gs reads the coefficient (and field) image of
+  // phantom gs / spp, pstride elements after phantom 0's; spp = 0: one image
+  int spp;
+  long long pstride;
 };
 hipError_t launch_forward2_edges(const Fwd2EdgeArgs &a, int nshots, hipStream_t st);
 
@@ -203,6 +219,10 @@ struct Adj2Args {
   const unsigned char *row_has_recv;
   int sponge_w;
   const float *sp_a, *sp_ab;
+  // stacked phantoms: shot gs reads the coefficient (and field) image of
+  // phantom gs / spp, pstride elements after phantom 0's; spp = 0: one image
+  int spp;
+  long long pstride;
 };
 // Three adjoint steps k+1, k+2 -> k -> k-1 -> k-2 in one pass over a strip
 // (same conditions as the two-step adjoint).  z(k) and z(k-1) are kept in

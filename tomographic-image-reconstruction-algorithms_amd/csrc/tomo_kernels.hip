@@ -88,11 +88,18 @@ __device__ __forceinline__ void edge_inward(int nx, int ny, int x, int y, int &i
   idb = jb * nx + ib;
 }
 
+// Offset of the coefficient / field image of global shot gs in a context that
+// stacks several phantoms of one geometry along the shot axis (spp shots per
+// phantom, images pstride elements apart); 0 for a single phantom (spp = 0).
+__device__ __forceinline__ long long phantom_offset(int spp, long long pstride, int gs) {
+  return spp > 0 ? (long long)(gs / spp) * pstride : 0;
+}
+
 template <typename T>
-__device__ T forward_edge(const FwdArgs<T> &A, const T *__restrict__ u0, const T *__restrict__ um,
+__device__ T forward_edge(const FwdArgs<T> &A, const float *__restrict__ f, const T *__restrict__ u0, const T *__restrict__ um,
                           int x, int y) {
   const int id = y * A.nx + x;
-  const T r = A.c0dt_h * sqrt(T(1) + T(A.f[id]));
+  const T r = A.c0dt_h * sqrt(T(1) + T(f[id]));
   int ida, idb;
   edge_inward(A.nx, A.ny, x, y, ida, idb);
   return mur_edge<T>(r, u0[id], u0[ida], u0[idb], um[id], um[ida]);
@@ -116,7 +123,8 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<
   const T *__restrict__ u0 = A.u0 + so;
   const T *__restrict__ um = A.um + so;
   T *__restrict__ up = A.up + so;
-  const T *__restrict__ pw = A.pw;
+  const long long po = phantom_offset(A.spp, A.pstride, gs);
+  const T *__restrict__ pw = A.pw + po;
   const bool xin = x > 0 && x < nx - 1;
 
   T c[RY + 2], l[RY], r[RY], p[RY], m[RY];
@@ -159,7 +167,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_step_kernel(FwdArgs<
       val = leapfrog<T>(p[i], sum, c[i + 1], a, ab, m[i]);
       if (src_col && y >= bb.z && y <= bb.w) val = fused(p[i], T(A.amp_h2), val);
     } else {
-      val = forward_edge(A, u0, um, x, y);
+      val = forward_edge(A, A.f + po, u0, um, x, y);
     }
     up[id] = val;
     if (cm[i] >= 0) {
@@ -187,11 +195,10 @@ __device__ __forceinline__ void sponge_factors(const AdjArgs &A, int x, int y, f
 // Interior adjoint stencil at (x, y) (backpropagation1, sirt-mimo.cu:686-710,
 // written with the per-cell weight pw = c0^2 dt^2 (1 + f) / h^2).
 template <bool SPONGE>
-__device__ float adjoint_interior(const AdjArgs &A, const float *__restrict__ z1,
+__device__ float adjoint_interior(const AdjArgs &A, const float *__restrict__ pw, const float *__restrict__ z1,
                                   const float *__restrict__ z2, int x, int y) {
   const int nx = A.nx;
   const int id = y * nx + x;
-  const float *__restrict__ pw = A.pw;
   float a, ab;
   sponge_factors<SPONGE>(A, x, y, a, ab);
   return adjoint_rule(pw[id - nx], z1[id - nx], pw[id + nx], z1[id + nx], pw[id - 1], z1[id - 1], pw[id + 1],
@@ -238,15 +245,16 @@ __device__ void adjoint_slow(const AdjArgs &A, const float *__restrict__ z1, con
                              const float *__restrict__ u, int gs, int x, int y, int ri, float &zval,
                              float &lu, float &uval) {
   const int nx = A.nx, ny = A.ny;
+  const float *__restrict__ pw = A.pw + phantom_offset(A.spp, A.pstride, gs);
   const bool ex = (x == 0 || x == nx - 1), ey = (y == 0 || y == ny - 1);
   const int xi = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
   const int yi = y == 0 ? 1 : (y == ny - 1 ? ny - 2 : y);
   if (ex || ey) {
-    zval = adjoint_interior<SPONGE>(A, z1, z2, xi, yi);
+    zval = adjoint_interior<SPONGE>(A, pw, z1, z2, xi, yi);
   } else {
     const int an = A.anchor[ri];
     zval = fused(A.rr[((long long)gs * A.nt + A.level) * A.nr + ri], A.gain,
-                 adjoint_interior<SPONGE>(A, z1, z2, an % nx, an / nx));
+                 adjoint_interior<SPONGE>(A, pw, z1, z2, an % nx, an / nx));
   }
   lu = edge_lu(u, nx, ny, x, y, A.inv_h2);
   uval = u_corner_mean(u, nx, ny, x, y);
@@ -273,7 +281,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_step_kernel(AdjArgs 
   float *__restrict__ z0 = A.z0 + so;
   float *__restrict__ acc = A.acc + gs * n2;
   float *__restrict__ cov = COV ? A.cov + gs * n2 : nullptr;
-  const float *__restrict__ pw = A.pw;
+  const float *__restrict__ pw = A.pw + phantom_offset(A.spp, A.pstride, gs);
   const bool xin = x > 0 && x < nx - 1;
 
   float zc[RY + 2], pc[RY + 2], uc[RY + 2];
@@ -565,7 +573,9 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
   const float *__restrict__ u0 = A.u0 + so;
   const float *__restrict__ um = A.um + so;
   float *__restrict__ up = A.up + so;
-  const float *__restrict__ pw = A.pw;
+  const long long po = phantom_offset(A.spp, A.pstride, gs);
+  const float *__restrict__ pw = A.pw + po;
+  const float *__restrict__ fld = A.f + po;
   const int4 bb = A.bbox[gs];
   const bool src = A.amp_h2 != 0.f;
   const int yend = min(y0 + RY, ny);  // rows of this strip: [y0, yend)
@@ -587,7 +597,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
     qu[j] = load_row_halo<VEC>(u0, y0 + 1 + j, x0, nx, uok(y0 + 1 + j));
     qm[j] = load_row<VEC>(um + at(y0 + j, sok(y0 + j)), sok(y0 + j));
     qp[j] = load_row<VEC>(pw + at(y0 + j, sok(y0 + j)), sok(y0 + j));
-    qf[j] = edge_f<VEC>(A.f, y0 + j, x0, nx, sok(y0 + j));
+    qf[j] = edge_f<VEC>(fld, y0 + j, x0, nx, sok(y0 + j));
   }
 
 #pragma unroll 1
@@ -601,7 +611,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward_vec_kernel(Launch<Fw
       nu[j] = load_row_halo<VEC>(u0, yu, x0, nx, uok(yu));
       nm[j] = load_row<VEC>(um + at(ys, sok(ys)), sok(ys));
       np[j] = load_row<VEC>(pw + at(ys, sok(ys)), sok(ys));
-      nf[j] = edge_f<VEC>(A.f, ys, x0, nx, sok(ys));
+      nf[j] = edge_f<VEC>(fld, ys, x0, nx, sok(ys));
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
@@ -761,7 +771,9 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
   const float *__restrict__ um = A.um + so;
   float *__restrict__ up1 = A.up1 + so;
   float *__restrict__ up2 = A.up2 + so;
-  const float *__restrict__ pw = A.pw;
+  const long long po = phantom_offset(A.spp, A.pstride, gs);
+  const float *__restrict__ pw = A.pw + po;
+  const float *__restrict__ fld = A.f + po;
   const int4 bb = A.bbox[gs];
   const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
   auto ld = [&](const float *__restrict__ base, int y) {
@@ -799,7 +811,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
     qn[j] = ld(u0, y <= r1 ? y + 1 : -1);
     qm[j] = ld(um, y <= r1 ? y : -1);
     qp[j] = ld(pw, y <= r1 ? y : -1);
-    qf[j] = edge_f<VEC>(A.f, y, x0, nx, y <= r1);
+    qf[j] = edge_f<VEC>(fld, y, x0, nx, y <= r1);
   }
   Row<VEC> pp = cc, v2 = cc, v1 = cc;
   float fp = 0.f;
@@ -815,7 +827,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
       nn[j] = ld(u0, y <= r1 ? y + 1 : -1);
       nm[j] = ld(um, y <= r1 ? y : -1);
       np[j] = ld(pw, y <= r1 ? y : -1);
-      nf[j] = edge_f<VEC>(A.f, y, x0, nx, y <= r1);
+      nf[j] = edge_f<VEC>(fld, y, x0, nx, y <= r1);
     }
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
@@ -828,12 +840,12 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
         // the inputs (loaded here: once per kernel, in the first and last
         // strips only); the strip owning the row stores it
         if (r == 0) {
-          v0 = edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(A.f, 0), A.c0dt_h, x0,
+          v0 = edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(fld, 0), A.c0dt_h, x0,
                                   nx);
           if (own(0)) store_row<VEC>(up1 + x0, v0);
         } else if (r == ny - 1) {
           v0 = edge_row_step<VEC>(ld(u0, ny - 1), ld(u0, ny - 2), ld(u0, ny - 3), ld(um, ny - 1), ld(um, ny - 2),
-                                  ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+                                  ld(fld, ny - 1), A.c0dt_h, x0, nx);
           if (own(ny - 1)) store_row<VEC>(up1 + (ny - 1) * nx + x0, v0);
         } else {
           v0 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, qm[j], qp[j], qf[j], r, x0, A.amp1, bb);
@@ -850,10 +862,10 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void forward2_kernel(Fwd2Args A) 
         }
         // rows 0 and ny-1 of u(k+2), by the strips owning rows 0 and ny-2
         if (r == 2 && own(0))
-          store_row<VEC>(up2 + x0, edge_row_step<VEC>(v2, v1, v0, ld(u0, 0), cu, ld(A.f, 0), A.c0dt_h, x0, nx));
+          store_row<VEC>(up2 + x0, edge_row_step<VEC>(v2, v1, v0, ld(u0, 0), cu, ld(fld, 0), A.c0dt_h, x0, nx));
         if (r == ny - 1 && own(ny - 2))
           store_row<VEC>(up2 + (ny - 1) * nx + x0,
-                         edge_row_step<VEC>(v0, v1, v2, cc, cu, ld(A.f, ny - 1), A.c0dt_h, x0, nx));
+                         edge_row_step<VEC>(v0, v1, v2, cc, cu, ld(fld, ny - 1), A.c0dt_h, x0, nx));
         // measured values of row r+1 (its descriptor was read a row ago)
         const float m1n = mv(dn, o1), m2n = mv(dn, o1 + A.nr);
         v2 = v1;
@@ -894,7 +906,7 @@ __global__ __launch_bounds__(256) void forward2_edge_kernel(Fwd2EdgeArgs A) {
   const int nx = A.nx;
   if (c.y < 0) {
     const int x = c.x % nx, y = c.x / nx;
-    const float r = A.c0dt_h * sqrtf(1.f + A.f[c.x]);
+    const float r = A.c0dt_h * sqrtf(1.f + A.f[phantom_offset(A.spp, A.pstride, gs) + c.x]);
     int ida, idb;
     edge_inward(nx, A.ny, x, y, ida, idb);
     const float *__restrict__ u0 = A.u0 + so;
@@ -958,7 +970,9 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
   float *__restrict__ up1 = A.up1 + so;
   float *__restrict__ up2 = A.up2 + so;
   float *__restrict__ up3 = A3.up3 + so;
-  const float *__restrict__ pw = A.pw;
+  const long long po = phantom_offset(A.spp, A.pstride, gs);
+  const float *__restrict__ pw = A.pw + po;
+  const float *__restrict__ fld = A.f + po;
   const int4 bb = A.bbox[gs];
   const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, A.sponge_w, s_spa, s_spab, true);
   auto ld = [&](const float *__restrict__ base, int y) {
@@ -992,7 +1006,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
   const Row<VEC> zero = ld(u0, -1);
   Row<VEC> cu = ld(u0, q1lo - 1), cc = ld(u0, q1lo), cn = ld(u0, q1lo + 1), qm = ld(um, q1lo);
   Row<VEC> p0 = ld(pw, q1lo), p1 = zero, p2 = zero;
-  float f0 = edge_f<VEC>(A.f, q1lo, x0, nx, true), f1 = 0.f, f2 = 0.f;
+  float f0 = edge_f<VEC>(fld, q1lo, x0, nx, true), f1 = 0.f, f2 = 0.f;
   Row<VEC> w1a = zero, w1b = zero, w2a = zero, w2b = zero;
 #pragma unroll 1
   for (int r = q1lo; r <= rmax; ++r) {
@@ -1000,16 +1014,16 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
     const Row<VEC> nn = ld(u0, r + 2 <= q1hi + 1 ? r + 2 : -1);
     const Row<VEC> nm = ld(um, r + 1 <= q1hi ? r + 1 : -1);
     const Row<VEC> np = ld(pw, r + 1 <= q1hi ? r + 1 : -1);
-    const float nf = edge_f<VEC>(A.f, r + 1, x0, nx, r + 1 <= q1hi);
+    const float nf = edge_f<VEC>(fld, r + 1, x0, nx, r + 1 <= q1hi);
     const int d2 = desc(r + 2);
     // level 1 at row r
     Row<VEC> v1 = zero;
     if (r <= q1hi) {
       if (r == 0) {
-        v1 = edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(A.f, 0), A.c0dt_h, x0, nx);
+        v1 = edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(fld, 0), A.c0dt_h, x0, nx);
       } else if (r == ny - 1) {
         v1 = edge_row_step<VEC>(ld(u0, ny - 1), ld(u0, ny - 2), ld(u0, ny - 3), ld(um, ny - 1), ld(um, ny - 2),
-                                ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+                                ld(fld, ny - 1), A.c0dt_h, x0, nx);
       } else {
         v1 = step_row<VEC>(A, ls, s_spa, s_spab, cu, cc, cn, qm, p0, f0, r, x0, A.amp1, bb);
         if (REC && A.rec1 && own(r)) record_lane<VEC>(v1, d0, m1, has_meas, A.rec + o1);
@@ -1040,24 +1054,24 @@ __global__ __launch_bounds__(kBlockX *kBlockY) __attribute__((amdgpu_waves_per_e
     }
     // edge rows of levels 2 and 3
     if (r == 2 && own(0)) {  // row 0 of level 2: level-1 rows 0, 1, 2 are w1a, w1b, v1
-      const Row<VEC> e = edge_row_step<VEC>(w1a, w1b, v1, ld(u0, 0), cu, ld(A.f, 0), A.c0dt_h, x0, nx);
+      const Row<VEC> e = edge_row_step<VEC>(w1a, w1b, v1, ld(u0, 0), cu, ld(fld, 0), A.c0dt_h, x0, nx);
       store_row<VEC>(up2 + x0, e);
       w2b = e;  // the window's row 0 (row r-2 after this iteration's shift)
     }
     if (r == 3 && own(0)) {  // row 0 of level 3: level-2 rows 0, 1, 2 are w2a, w2b, v2; level-1 row 1 is w1a
       const Row<VEC> l1top =
-          edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(A.f, 0), A.c0dt_h, x0, nx);
-      store_row<VEC>(up3 + x0, edge_row_step<VEC>(w2a, w2b, v2, l1top, w1a, ld(A.f, 0), A.c0dt_h, x0, nx));
+          edge_row_step<VEC>(ld(u0, 0), ld(u0, 1), ld(u0, 2), ld(um, 0), ld(um, 1), ld(fld, 0), A.c0dt_h, x0, nx);
+      store_row<VEC>(up3 + x0, edge_row_step<VEC>(w2a, w2b, v2, l1top, w1a, ld(fld, 0), A.c0dt_h, x0, nx));
     }
     if (r == ny - 1 && q2hi == ny - 1) {  // row ny-1 of level 2: level-1 rows ny-1, ny-2, ny-3 are v1, w1b, w1a
-      const Row<VEC> e = edge_row_step<VEC>(v1, w1b, w1a, cc, cu, ld(A.f, ny - 1), A.c0dt_h, x0, nx);
+      const Row<VEC> e = edge_row_step<VEC>(v1, w1b, w1a, cc, cu, ld(fld, ny - 1), A.c0dt_h, x0, nx);
 #pragma unroll
       for (int i = 0; i < VEC; ++i) s_e2[threadIdx.y][i][threadIdx.x] = e.e[i];
       if (own(ny - 1)) store_row<VEC>(up2 + (ny - 1) * nx + x0, e);
     }
     if (r == ny && own(ny - 1)) {  // row ny-1 of level 3: level-2 rows ny-1, ny-2, ny-3 are v2, w2b, w2a
       store_row<VEC>(up3 + (ny - 1) * nx + x0,
-                     edge_row_step<VEC>(v2, w2b, w2a, w1b, w1a, ld(A.f, ny - 1), A.c0dt_h, x0, nx));
+                     edge_row_step<VEC>(v2, w2b, w2a, w1b, w1a, ld(fld, ny - 1), A.c0dt_h, x0, nx));
     }
     // measured values for the next iteration: level 1 at row r+1, level 2
     // at row r, level 3 at row r-1
@@ -1108,7 +1122,7 @@ __global__ __launch_bounds__(256) void forward_ring_kernel(Launch<FwdArgs<float>
   const int x = c.x % A.nx, y = c.x / A.nx;
   float val;
   if (c.y < 0) {
-    val = forward_edge(A, A.u0 + so, A.um + so, x, y);
+    val = forward_edge(A, A.f + phantom_offset(A.spp, A.pstride, gs), A.u0 + so, A.um + so, x, y);
     A.up[so + c.x] = val;
   } else {
     val = A.up[so + c.x];
@@ -1177,7 +1191,7 @@ __global__ __launch_bounds__(kBlockX *kBlockY) void adjoint_vec_kernel(Launch<Ad
   float *__restrict__ z0 = A.z0 + so;
   float *__restrict__ acc = A.acc + gs * n2;
   float *__restrict__ cov = COV ? A.cov + gs * n2 : nullptr;
-  const float *__restrict__ pw = A.pw;
+  const float *__restrict__ pw = A.pw + phantom_offset(A.spp, A.pstride, gs);
   const int yend = min(y0 + RY, ny);
   const LaneSponge<VEC> ls = lane_sponge<VEC>(x0, nx, SPONGE ? A.sponge_w : 0, s_spa, s_spab, xok);
 
@@ -1496,7 +1510,7 @@ __device__ __forceinline__ void adjoint2_strip(const Adj2Args &A, const float2 *
   float *__restrict__ z0 = A.z0 + so;
   float *__restrict__ zm = A.zm + so;
   float *__restrict__ acc = A.acc + gs * n2;
-  const float *__restrict__ pw = A.pw;
+  const float *__restrict__ pw = A.pw + phantom_offset(A.spp, A.pstride, gs);
   const float *__restrict__ rrk = A.rr + ((long long)gs * A.nt + A.level) * A.nr;  // residuals at level k
   const float *__restrict__ rrm = rrk - A.nr;                                        // and at level k-1
   auto ld = [&](const float *__restrict__ base, int y, bool ok) {
@@ -1684,7 +1698,7 @@ __device__ __forceinline__ void adjoint3_strip(const Adj3Args &A3, const float2 
   float *__restrict__ zm = A.zm + so;
   float *__restrict__ zm2 = A3.zm2 + so;
   float *__restrict__ acc = A.acc + gs * n2;
-  const float *__restrict__ pw = A.pw;
+  const float *__restrict__ pw = A.pw + phantom_offset(A.spp, A.pstride, gs);
   const float *__restrict__ rrk = A.rr + ((long long)gs * A.nt + A.level) * A.nr;  // residuals at level k
   const float *__restrict__ rrm = rrk - A.nr;                                        // level k-1
   const float *__restrict__ rrm2 = rrm - A.nr;                                       // level k-2
