@@ -2001,23 +2001,29 @@ int tomo_autotune(tomo_ctx *c, int force, int *out5, double *ms2) {
         t.b.um = pass1_level(c, P, 1);
         t.um2 = pass1_level(c, P, 2);
         t.b.level = mid;
+        // strip height and read-ahead depth of that kernel (the one-step
+        // sweep's depth does not carry over to it)
         double b3 = 1e30;
-        int br3 = 0;
+        int br3 = 0, bd3 = 1;
         for (int rows : {16, 32, 64}) {
           if (rows > c->ny) continue;
-          double t3 = 0.0;
-          rc = time_launches(c, reps, [&]() -> int {
-            HIPCHK(c, launch_adjoint3(t, P, rows, 1, c->stream));
-            return TOMO_OK;
-          }, &t3);
-          if (rc) {
-            c->shape_req = keep;
-            return rc;
+          for (int depth : {1, 2}) {
+            if (rows == 64 && depth == 2) continue;  // 64-row strips read one row ahead only
+            double t3 = 0.0;
+            rc = time_launches(c, reps, [&]() -> int {
+              HIPCHK(c, launch_adjoint3(t, P, rows, depth, c->stream));
+              return TOMO_OK;
+            }, &t3);
+            if (rc) {
+              c->shape_req = keep;
+              return rc;
+            }
+            if (t3 < b3) { b3 = t3; br3 = rows; bd3 = depth; }
           }
-          if (t3 < b3) { b3 = t3; br3 = rows; }
         }
         if (br3) {
           best.shape.adj_rows = br3;
+          best.shape.adj_depth = bd3;
           best.adj_ms = b3 / 3;
         }
       }

@@ -11,8 +11,9 @@ f_p = 0.1, point receivers at the transducers).
   inclusions, forward wavefield checkpointed every 200 steps.
 - ``c4``: 64 phantoms, sides drawn uniformly from {128, 192, 256, 384, 512},
   n/4 transducers, 8 n steps, inclusions as in c1; checkpoint every 128
-  steps, 32 shots per pass (or all shots in one pass where the engines
-  share wavefield workspaces).
+  steps.  Either one engine per phantom (32 shots per pass, or all shots in
+  one pass where the engines share wavefield workspaces), or one engine per
+  side with that side's phantoms stacked along the shot axis.
 
 Ring radius 0.39 n unless stated; transducers fire one at a time; seeds are
 fixed per workload.
@@ -97,3 +98,51 @@ def c4_problems(count: int = 64, seed: int = 4000, device: int = 0, log=None, pa
         if log:
             log(f"c4 problem {i}: {n}x{n}, {n // 4} shots, {8 * n} steps")
     return engines
+
+
+def c4_stacks(count: int = 64, seed: int = 4000, device: int = 0, log=None, max_pass_bytes: float = 190e9):
+    """The uneven batch as one engine per side, that side's phantoms stacked
+    along the shot axis (tomo_set_phantoms: every stencil launch advances
+    every shot of every phantom of the side).  Same phantoms, geometry and
+    FP64 data as c4_problems.  Shots per pass: all of them, or as many as
+    keep the wavefield storage (checkpoint pairs, the replayed segment, the
+    adjoint ring) within max_pass_bytes.  Returns [(side, engine, phantom
+    indices)]."""
+    sides = c4_sides(count, seed)
+    out = []
+    for n in C4_SIDES:
+        idx = [i for i, v in enumerate(sides) if v == n]
+        if not idx:
+            continue
+        c = config_unit(n, 8 * n + 2)
+        c.checkpoint_every = 128
+        shots = n // 4
+        levels = 2 * ((c.nt + 127) // 128) + 128 + 5
+        fit = int(max_pass_bytes // (levels * n * n * 4))
+        total = shots * len(idx)
+        # equal passes, each a whole number of 128-shot rounds of waves where
+        # the storage allows (a 512^2 pass of 896 shots: 7 full rounds of the
+        # three-step adjoint's one wave per SIMD)
+        passes = -(-total // max(fit, 1))
+        per = -(-total // passes)
+        if passes > 1 and (-(-per // 128)) * 128 <= fit:
+            per = -(-per // 128) * 128
+        c.shots_per_pass = 0 if passes <= 1 else per
+        ii, jj = transducers_ring(n, n, shots, 0.39 * n)
+        fos = [c1_phantom(n, seed + 1 + i, (c.mask_x0, c.mask_x1, c.mask_y0, c.mask_y1)) for i in idx]
+        e = Engine(c, device=device)
+        e.set_phantoms(len(idx))
+        e.set_transducers(ii, jj, one_based=False)
+        e.set_receivers(ii, jj, one_based=False)
+        e.set_groups(1)
+        truth = np.concatenate(fos)
+        e.set_truth(truth)
+        e.set_field(truth)
+        meas = e.forward(fp64=True)
+        e.set_field(np.zeros(truth.size, np.float32))
+        e.set_measured(meas)
+        out.append((n, e, idx))
+        if log:
+            log(f"c4 side {n}: {len(idx)} phantoms stacked, {total} shots, {8 * n} steps, "
+                f"shots per pass {c.shots_per_pass or total}")
+    return out
