@@ -267,7 +267,11 @@ def main():
     ap.add_argument("--phantoms", type=int, default=48, help="training phantoms of --study train")
     ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of --study train")
     ap.add_argument("--train-target", default="best",
-                    help="--study train targets: best (the trajectory's best iterate) or ahead:K (K iterations on)")
+                    help="--study train targets: best (the trajectory's best iterate), ahead:K (K iterations on) "
+                         "or truth (the true field)")
+    ap.add_argument("--train-rounds", type=int, default=0,
+                    help="--study train: rounds that add the learned iteration's own states (target: the truth) "
+                         "and train on")
     ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
     ap.add_argument("--learned-iters", type=int, default=3,
                     help="--study ttt: learned iterations (with the plain fallback) before plain ones")
@@ -305,7 +309,7 @@ def main():
             out = S.c2_study(log=log)
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
-                                target=args.train_target, log=log)
+                                target=args.train_target, rounds=args.train_rounds, log=log)
         elif args.study == "finetune":
             out = S.finetune_study(args.out, base=args.weights, phantoms=args.phantoms,
                                    train_steps=args.train_steps, log=log)

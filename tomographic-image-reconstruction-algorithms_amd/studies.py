@@ -105,13 +105,15 @@ def _mask(c):
     return m.ravel()
 
 
-def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: str = "best", log=None) -> dict:
+def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: str = "best", rounds: int = 0,
+                log=None) -> dict:
     """Reproduce the shipped network (see the module docstring; `phantoms`
     seeded phantoms from 7000); writes the parameters (float32, numpy .npy)
-    to `out`."""
+    to `out`.  rounds > 0 adds that many rounds of the learned iteration's
+    own states (target: the truth) after the plain trajectories."""
     net = Preconditioner(device=0, seed=1)
     t = time.perf_counter()
-    rep = L.train(net, 256, range(7000, 7000 + phantoms), TRAIN_ITERS, frac=TTT_FRAC, rounds=0,
+    rep = L.train(net, 256, range(7000, 7000 + phantoms), TRAIN_ITERS, frac=TTT_FRAC, rounds=rounds,
                   train_steps=train_steps, batch=8, lr=1e-3, target=target, fp=TTT_FP, log=log)
     np.save(out, net.get_params(), allow_pickle=False)
     net.close()
