@@ -260,7 +260,7 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel from the host (default: each gradient pass is captured once into a "
                          "HIP graph and replayed)")
-    ap.add_argument("--study", choices=["c2", "ttt", "train", "finetune"], default=None,
+    ap.add_argument("--study", choices=["c2", "ttt", "train", "finetune", "relax"], default=None,
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
     ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
     ap.add_argument("--out", default="preconditioner_trained.npy", help="parameters written by --study train")
@@ -307,6 +307,8 @@ def main():
         log = (lambda m: print("[study]", m, file=sys.stderr, flush=True))
         if args.study == "c2":
             out = S.c2_study(log=log)
+        elif args.study == "relax":
+            out = S.relax_study(seed=int(args.seeds.split(",")[0]), log=log)
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
                                 target=args.train_target, rounds=args.train_rounds, log=log)

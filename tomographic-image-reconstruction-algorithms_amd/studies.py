@@ -33,6 +33,10 @@
   at relaxation 0.3 x the misfit-optimal first step, every iterate a sample
   whose target is the trajectory's best iterate (lowest error), 4000 Adam
   steps of batch 8 at learning rate 1e-3.
+- ``relax``: how fast plain SIRT itself can go on one held-out ttt phantom:
+  50 iterations each of the uniform update at 1, 2 and 3 x the ttt
+  relaxation, and at 1 x with per-pixel coverage (illumination) weights of
+  eps 0.05 and 0.2.
 - ``finetune``: continues training a network on plain-SIRT trajectories at
   the evaluation size, 512^2 with 128 transducers and 4000 steps (seeds from
   8000, disjoint from the held-out 9100-9107), targets as in ``train``.
@@ -205,3 +209,27 @@ def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = TTT_F
         "speedup_fixed50": res["fixed50"]["speedup"], "speedup_stop_rule": res["stop_rule"]["speedup"],
         "speedup_fixed10": res["fixed10"]["speedup"],
     }
+
+
+def relax_study(seed: int = 9101, iters: int = 50, fp: float = TTT_FP, log=None) -> dict:
+    """Plain SIRT on the ttt problem of `seed` at several relaxations and with
+    coverage weights (see the module docstring): the error trajectory of each
+    variant and its wall seconds."""
+    n = 512
+    p = L.Problem(n, L.phantom(n, seed), steps=4000, shots=128, checkpoint=200, fp=fp)
+    scale = L.calibrate_scale(p, TTT_FRAC)
+    rows = []
+    for name, weights, eps_w, mult in [("uniform x1", 0, 0.05, 1.0), ("coverage eps 0.05", 1, 0.05, 1.0),
+                                       ("coverage eps 0.2", 1, 0.2, 1.0), ("uniform x2", 0, 0.05, 2.0),
+                                       ("uniform x3", 0, 0.05, 3.0)]:
+        p.reset()
+        p.engine.enable_coverage(weights == 1)
+        p.engine.set_pixel_weights(weights, eps_w)
+        tr = L.run_plain(p, scale * mult, iters)
+        rows.append({"variant": name, "eps": [round(v, 3) for v in tr.eps], "seconds": round(tr.seconds[-1], 2)})
+        if log:
+            log(f"{name}: eps after 10 / 25 / {iters} iterations {tr.eps[9]:.2f} / {tr.eps[24]:.2f} / "
+                f"{tr.eps[-1]:.2f} in {tr.seconds[-1]:.1f} s")
+    p.close()
+    return {"study": "relax", "grid": n, "shots": 128, "time_steps": 4000, "ricker_fp": fp, "phantom_seed": seed,
+            "relaxation": scale, "relaxation_frac": TTT_FRAC, "rows": rows}
