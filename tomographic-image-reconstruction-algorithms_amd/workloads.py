@@ -100,6 +100,25 @@ def c4_problems(count: int = 64, seed: int = 4000, device: int = 0, log=None, pa
     return engines
 
 
+def stack_pass_shots(n: int, nt: int, checkpoint: int, total: int, max_pass_bytes: float) -> int:
+    """Shots per pass of a stacked engine (0: all `total` in one pass): the
+    wavefield storage of a pass (two checkpoint levels per segment, the
+    replayed segment and the 5-level adjoint ring, n^2 floats per shot and
+    level) stays within max_pass_bytes, the passes are equal, and each is a
+    whole number of 128-shot rounds of waves where the storage allows (a
+    512^2 pass of 896 shots: 7 full rounds of the three-step adjoint's one
+    wave per SIMD)."""
+    levels = 2 * (-(-nt // checkpoint)) + checkpoint + 5
+    fit = max(int(max_pass_bytes // (levels * n * n * 4)), 1)
+    passes = -(-total // fit)
+    if passes <= 1:
+        return 0
+    per = -(-total // passes)
+    if -(-per // 128) * 128 <= fit:
+        per = -(-per // 128) * 128
+    return per
+
+
 def c4_stacks(count: int = 64, seed: int = 4000, device: int = 0, log=None, max_pass_bytes: float = 190e9):
     """The uneven batch as one engine per side, that side's phantoms stacked
     along the shot axis (tomo_set_phantoms: every stencil launch advances
@@ -117,17 +136,8 @@ def c4_stacks(count: int = 64, seed: int = 4000, device: int = 0, log=None, max_
         c = config_unit(n, 8 * n + 2)
         c.checkpoint_every = 128
         shots = n // 4
-        levels = 2 * ((c.nt + 127) // 128) + 128 + 5
-        fit = int(max_pass_bytes // (levels * n * n * 4))
         total = shots * len(idx)
-        # equal passes, each a whole number of 128-shot rounds of waves where
-        # the storage allows (a 512^2 pass of 896 shots: 7 full rounds of the
-        # three-step adjoint's one wave per SIMD)
-        passes = -(-total // max(fit, 1))
-        per = -(-total // passes)
-        if passes > 1 and (-(-per // 128)) * 128 <= fit:
-            per = -(-per // 128) * 128
-        c.shots_per_pass = 0 if passes <= 1 else per
+        c.shots_per_pass = stack_pass_shots(n, c.nt, 128, total, max_pass_bytes)
         ii, jj = transducers_ring(n, n, shots, 0.39 * n)
         fos = [c1_phantom(n, seed + 1 + i, (c.mask_x0, c.mask_x1, c.mask_y0, c.mask_y1)) for i in idx]
         e = Engine(c, device=device)
