@@ -263,6 +263,7 @@ def main():
     ap.add_argument("--study", choices=["c2", "ttt", "train", "finetune", "relax"], default=None,
                     help="run a convergence study instead of the throughput benchmark (one JSON line)")
     ap.add_argument("--seeds", default="9100", help="held-out phantom seeds of the ttt study (comma separated)")
+    ap.add_argument("--fp", type=float, default=0.0, help="--study c2: Ricker peak frequency (default 0.1)")
     ap.add_argument("--out", default="preconditioner_trained.npy", help="parameters written by --study train")
     ap.add_argument("--phantoms", type=int, default=48, help="training phantoms of --study train")
     ap.add_argument("--train-steps", type=int, default=4000, help="Adam steps of --study train")
@@ -306,7 +307,7 @@ def main():
         import tomo_amd.studies as S
         log = (lambda m: print("[study]", m, file=sys.stderr, flush=True))
         if args.study == "c2":
-            out = S.c2_study(log=log)
+            out = S.c2_study(fp=args.fp or 0.1, log=log)
         elif args.study == "relax":
             out = S.relax_study(seed=int(args.seeds.split(",")[0]), log=log)
         elif args.study == "train":
