@@ -270,6 +270,9 @@ def main():
     ap.add_argument("--train-target", default="best",
                     help="--study train targets: best (the trajectory's best iterate), ahead:K (K iterations on) "
                          "or truth (the true field)")
+    ap.add_argument("--train-iters", type=int, default=50, help="--study train: plain iterations per phantom")
+    ap.add_argument("--stack", type=int, default=0,
+                    help="--study train: phantoms per engine, stacked along the shot axis (0: one engine each)")
     ap.add_argument("--train-rounds", type=int, default=0,
                     help="--study train: rounds that add the learned iteration's own states (target: the truth) "
                          "and train on")
@@ -312,7 +315,8 @@ def main():
             out = S.relax_study(seed=int(args.seeds.split(",")[0]), log=log)
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
-                                target=args.train_target, rounds=args.train_rounds, log=log)
+                                target=args.train_target, rounds=args.train_rounds, iters=args.train_iters,
+                                stack=args.stack, log=log)
         elif args.study == "finetune":
             out = S.finetune_study(args.out, base=args.weights, phantoms=args.phantoms,
                                    train_steps=args.train_steps, log=log)

@@ -532,6 +532,16 @@ int tomo_cnn_loss(tomo_cnn *net, int batch, int h, int w, const float *x, const 
  * (tomo_gradient) and the coverage, target from its truth; all samples share
  * one grid size and mask */
 int tomo_cnn_add_sample(tomo_cnn *net, tomo_ctx *ctx, float scale);
+/* the same for phantom p of a context of stacked phantoms
+ * (tomo_set_phantoms): its field, the plain step from its own shots'
+ * gradients, its coverage and its truth; one call per phantom collects the
+ * samples of many phantoms from one batched gradient.  No reference
+ * counterpart. */
+int tomo_cnn_add_sample_phantom(tomo_cnn *net, tomo_ctx *ctx, int p, float scale);
+/* storage for `count` samples of ny x nx cells, allocated once (samples
+ * already held are kept), so that collecting many samples does not regrow
+ * it step by step */
+int tomo_cnn_reserve_samples(tomo_cnn *net, int count, int ny, int nx);
 int tomo_cnn_num_samples(const tomo_cnn *net);
 /* new targets (field - f_s) / 0.1 for samples [first, first + count), f_s the
  * field each was taken at (e.g. field = a later iterate of the trajectory) */

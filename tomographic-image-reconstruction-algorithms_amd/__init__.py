@@ -158,6 +158,8 @@ SIGNATURES = {
     "tomo_cnn_backward": (_int, [_vp, _int, _int, _int, _f32p, _f32p, _f32p]),
     "tomo_cnn_loss": (_int, [_vp, _int, _int, _int, _f32p, _f32p, _i32p, ctypes.POINTER(ctypes.c_double), _vp]),
     "tomo_cnn_add_sample": (_int, [_vp, _vp, ctypes.c_float]),
+    "tomo_cnn_add_sample_phantom": (_int, [_vp, _vp, _int, ctypes.c_float]),
+    "tomo_cnn_reserve_samples": (_int, [_vp, _int, _int, _int]),
     "tomo_cnn_num_samples": (_int, [_vp]),
     "tomo_cnn_set_targets": (_int, [_vp, _int, _int, _f32p]),
     "tomo_cnn_clear_samples": (_int, [_vp]),
@@ -667,8 +669,17 @@ class Preconditioner:
         self._c(lib().tomo_cnn_loss(self._net, B, H, W, x, _f32(target), _i32(mask4), ctypes.byref(L), _ptr(g)))
         return (L.value, g) if grad else L.value
 
-    def add_sample(self, engine: "Engine", scale: float):
-        self._c(lib().tomo_cnn_add_sample(self._net, engine._ctx, float(scale)))
+    def add_sample(self, engine: "Engine", scale: float, phantom: int | None = None):
+        """One training sample from the engine's state (phantom `phantom` of a
+        context of stacked phantoms)."""
+        if phantom is None:
+            self._c(lib().tomo_cnn_add_sample(self._net, engine._ctx, float(scale)))
+        else:
+            self._c(lib().tomo_cnn_add_sample_phantom(self._net, engine._ctx, int(phantom), float(scale)))
+
+    def reserve_samples(self, count: int, ny: int, nx: int):
+        """Storage for `count` samples of ny x nx cells, allocated once."""
+        self._c(lib().tomo_cnn_reserve_samples(self._net, int(count), int(ny), int(nx)))
 
     def num_samples(self) -> int:
         return int(lib().tomo_cnn_num_samples(self._net))

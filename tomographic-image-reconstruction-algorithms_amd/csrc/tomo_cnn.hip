@@ -539,6 +539,7 @@ struct tomo_cnn {
   size_t tmp3_n = 0;
   // training samples (all of one grid size)
   int sH = 0, sW = 0, nsamples = 0, cap_samples = 0;
+  size_t cap_n2 = 0;  // cells per sample the storage was sized for
   Mask smask{0, 0, 0, 0};
   __bf16 *d_sx = nullptr;   // [N][H][W][8]
   float *d_st = nullptr;    // [N][H][W]  targets
@@ -747,29 +748,57 @@ int ensure_engine_scratch(tomo_cnn *n, long long n2) {
 
 // network input of the context's current state into d_x8 (and the target
 // into `target` when non-null); needs gradients of every shot
-int engine_input(tomo_cnn *n, tomo_ctx *c, float scale, float *target) {
+// sample storage for `cap` samples of n2 cells (the stored ones are kept)
+int reserve_samples(tomo_cnn *n, int cap, size_t n2) {
+  int rc;
+  __bf16 *sx = nullptr;
+  float *st = nullptr, *sf = nullptr;
+  if ((rc = nalloc(n, &sx, (size_t)cap * n2 * 8)) || (rc = nalloc(n, &st, (size_t)cap * n2)) ||
+      (rc = nalloc(n, &sf, (size_t)cap * n2)))
+    return rc;
+  if (n->nsamples) {
+    CNNCHK(n, hipMemcpy(sx, n->d_sx, sizeof(__bf16) * n->nsamples * n2 * 8, hipMemcpyDeviceToDevice));
+    CNNCHK(n, hipMemcpy(st, n->d_st, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
+    CNNCHK(n, hipMemcpy(sf, n->d_sf, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
+  }
+  if (n->d_sx) (void)hipFree(n->d_sx);
+  if (n->d_st) (void)hipFree(n->d_st);
+  if (n->d_sf) (void)hipFree(n->d_sf);
+  n->d_sx = sx;
+  n->d_st = st;
+  n->d_sf = sf;
+  n->cap_samples = cap;
+  n->cap_n2 = n2;
+  return TOMO_OK;
+}
+
+// network input of phantom p of the context (its field, its own shots'
+// plain step and coverage, its truth for the target)
+int engine_input(tomo_cnn *n, tomo_ctx *c, int p, float scale, float *target) {
   int rc;
   if ((rc = ensure_engine_scratch(n, c->n2))) return rc;
   if ((rc = ensure_batch(n, (size_t)c->n2))) return rc;
   const Mask m = ctx_mask(c);
+  const int G = c->nshots / c->nph;
+  const float *f = c->d_f + (size_t)p * c->n2, *fo = c->d_fo + (size_t)p * c->n2;
+  const float *grad = c->d_grad + (size_t)p * G * c->n2;
   // the engine stream orders these after the gradient pass; the network
   // stream waits on a sync below
-  hipLaunchKernelGGL(direction_kernel, dim3(blocks_for(c->n2)), dim3(256), 0, c->stream, (const float *)c->d_f,
-                     (const float *)c->d_grad, c->nshots, c->nx, c->ny, m, n->d_dir);
+  hipLaunchKernelGGL(direction_kernel, dim3(blocks_for(c->n2)), dim3(256), 0, c->stream, f, grad, G, c->nx, c->ny,
+                     m, n->d_dir);
   HIPCHK(c, hipGetLastError());
   const float *cov = nullptr;
   if (c->coverage && c->d_cov) {
     hipLaunchKernelGGL(coverage_sum_kernel, dim3(blocks_for(c->n2)), dim3(256), 0, c->stream,
-                       (const float *)c->d_cov, c->nshots, c->n2, n->d_cov);
+                       (const float *)(c->d_cov + (size_t)p * G * c->n2), G, c->n2, n->d_cov);
     HIPCHK(c, hipGetLastError());
     cov = n->d_cov;
   }
   hipLaunchKernelGGL(input_stats_kernel, dim3(1), dim3(256), 0, c->stream, (const float *)n->d_dir, cov, c->nx,
                      c->ny, m, n->d_red);
   HIPCHK(c, hipGetLastError());
-  hipLaunchKernelGGL(make_input_kernel, dim3(blocks_for(c->n2)), dim3(256), 0, c->stream, (const float *)c->d_f,
-                     (const float *)n->d_dir, scale, cov, (const double *)n->d_red, (int)c->n2, n->d_x8,
-                     (const float *)c->d_fo, target);
+  hipLaunchKernelGGL(make_input_kernel, dim3(blocks_for(c->n2)), dim3(256), 0, c->stream, f,
+                     (const float *)n->d_dir, scale, cov, (const double *)n->d_red, (int)c->n2, n->d_x8, fo, target);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TOMO_OK;
@@ -929,11 +958,11 @@ int tomo_cnn_clear_samples(tomo_cnn *n) {
 
 int tomo_cnn_num_samples(const tomo_cnn *n) { return n ? n->nsamples : 0; }
 
-int tomo_cnn_add_sample(tomo_cnn *n, tomo_ctx *c, float scale) {
+int tomo_cnn_add_sample_phantom(tomo_cnn *n, tomo_ctx *c, int p, float scale) {
   if (!n || !c) return TOMO_ERR_ARG;
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (c->nph != 1) return cfail(n, TOMO_ERR_STATE, "tomo_cnn_add_sample takes a context of one phantom");
+  if (p < 0 || p >= c->nph) return cfail(n, TOMO_ERR_ARG, "phantom index out of range");
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return cfail(n, rc, c->err);
   if (!c->have_truth) return cfail(n, TOMO_ERR_STATE, "the context has no ground truth");
   if (c->nx % 32 != 0) return cfail(n, TOMO_ERR_ARG, "grid width must be a multiple of 32");
@@ -947,34 +976,33 @@ int tomo_cnn_add_sample(tomo_cnn *n, tomo_ctx *c, float scale) {
     return cfail(n, TOMO_ERR_ARG, "samples must share one grid size and mask");
   }
   const size_t n2 = (size_t)c->n2;
-  if (n->nsamples == n->cap_samples) {
-    const int cap = std::max(16, n->cap_samples * 2);
-    __bf16 *sx = nullptr;
-    float *st = nullptr, *sf = nullptr;
-    if ((rc = nalloc(n, &sx, (size_t)cap * n2 * 8)) || (rc = nalloc(n, &st, (size_t)cap * n2)) ||
-        (rc = nalloc(n, &sf, (size_t)cap * n2)))
-      return rc;
-    if (n->nsamples) {
-      CNNCHK(n, hipMemcpy(sx, n->d_sx, sizeof(__bf16) * n->nsamples * n2 * 8, hipMemcpyDeviceToDevice));
-      CNNCHK(n, hipMemcpy(st, n->d_st, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
-      CNNCHK(n, hipMemcpy(sf, n->d_sf, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
-    }
-    if (n->d_sx) (void)hipFree(n->d_sx);
-    if (n->d_st) (void)hipFree(n->d_st);
-    if (n->d_sf) (void)hipFree(n->d_sf);
-    n->d_sx = sx;
-    n->d_st = st;
-    n->d_sf = sf;
-    n->cap_samples = cap;
-  }
-  if ((rc = engine_input(n, c, scale, n->d_st + (size_t)n->nsamples * n2))) return rc;
+  // storage reserved for another grid size holds no sample yet: resize it
+  if (n->nsamples == 0 && n->cap_samples > 0 && n->cap_n2 != n2 && (rc = reserve_samples(n, n->cap_samples, n2)))
+    return rc;
+  if (n->nsamples == n->cap_samples && (rc = reserve_samples(n, std::max(16, n->cap_samples * 2), n2))) return rc;
+  if ((rc = engine_input(n, c, p, scale, n->d_st + (size_t)n->nsamples * n2))) return rc;
   CNNCHK(n, hipMemcpyAsync(n->d_sx + (size_t)n->nsamples * n2 * 8, n->d_x8, sizeof(__bf16) * n2 * 8,
                            hipMemcpyDeviceToDevice, c->stream));
-  CNNCHK(n, hipMemcpyAsync(n->d_sf + (size_t)n->nsamples * n2, c->d_f, sizeof(float) * n2, hipMemcpyDeviceToDevice,
+  CNNCHK(n, hipMemcpyAsync(n->d_sf + (size_t)n->nsamples * n2, c->d_f + (size_t)p * n2, sizeof(float) * n2, hipMemcpyDeviceToDevice,
                            c->stream));
   CNNCHK(n, hipStreamSynchronize(c->stream));
   n->nsamples += 1;
   return TOMO_OK;
+}
+
+int tomo_cnn_add_sample(tomo_cnn *n, tomo_ctx *c, float scale) {
+  if (!n || !c) return TOMO_ERR_ARG;
+  if (c->nph != 1) return cfail(n, TOMO_ERR_STATE, "tomo_cnn_add_sample takes a context of one phantom");
+  return tomo_cnn_add_sample_phantom(n, c, 0, scale);
+}
+
+int tomo_cnn_reserve_samples(tomo_cnn *n, int count, int ny, int nx) {
+  if (!n || count < 0 || ny <= 0 || nx <= 0) return TOMO_ERR_ARG;
+  if (n->nsamples > 0 && (n->sH != ny || n->sW != nx)) return cfail(n, TOMO_ERR_ARG, "samples must share one grid size");
+  const size_t n2 = (size_t)ny * nx;
+  if (count <= n->cap_samples && n2 == n->cap_n2) return TOMO_OK;
+  CNNCHK(n, hipSetDevice(n->device));
+  return reserve_samples(n, std::max(count, n->nsamples), n2);
 }
 
 int tomo_cnn_set_targets(tomo_cnn *n, int first, int count, const float *field) {
@@ -1042,7 +1070,7 @@ int tomo_learned_update(tomo_ctx *c, tomo_cnn *n, float alpha, float scale) {
   if (c->nph != 1) return fail(c, TOMO_ERR_STATE, "tomo_learned_update takes a context of one phantom");
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
   if (c->nx % 32 != 0) return fail(c, TOMO_ERR_ARG, "grid width must be a multiple of 32");
-  if ((rc = engine_input(n, c, scale, nullptr))) return fail(c, rc, n->err);
+  if ((rc = engine_input(n, c, 0, scale, nullptr))) return fail(c, rc, n->err);
   if ((rc = forward_net(n, 1, c->ny, c->nx))) return fail(c, rc, n->err);
   CNNCHK(n, hipStreamSynchronize(n->stream));
   hipLaunchKernelGGL(learned_update_kernel, dim3(blocks_for(c->n2)), dim3(256), 0, c->stream, c->d_f,

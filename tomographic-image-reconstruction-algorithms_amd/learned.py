@@ -19,6 +19,12 @@ misfit along the first plain direction (doubling search, forward-only
 misfits) and returns ``frac * s*``; the same relaxation drives the plain
 iteration, the trajectories the network is trained on, and the learned
 iteration's fallback.
+
+Training data at scale (``train_stacked``): phantoms of one size are stacked
+along the shot axis of one engine (``Engine.set_phantoms``), so one batched
+gradient per iteration advances the plain-SIRT trajectories of all of them,
+and every phantom's state becomes a sample (``Preconditioner.add_sample``
+with its phantom index).
 """
 from __future__ import annotations
 
@@ -292,4 +298,123 @@ def train(net: Preconditioner, n: int, seeds, iters: int, steps: int = 0, shots:
     rep.samples = net.num_samples()
     rep.rounds = rounds
     rep.seconds = {"trajectories": t_data, "training": t_train}
+    return rep
+
+
+def stacked_problem(n: int, seeds, steps: int = 0, shots: int = 0, checkpoint: int = -1, fp: float = 0.1,
+                    kind: str = "smooth", device: int = 0) -> Engine:
+    """One engine holding the phantoms `seeds` (same geometry as Problem),
+    stacked along the shot axis, with their FP64 data, truths and f = 0."""
+    steps = steps or 8 * n
+    shots = shots or n // 4
+    cfg = config_unit(n, steps + 2)
+    cfg.ricker_fp = fp
+    cfg.ricker_t0 = 1.5 / fp
+    cfg.checkpoint_every = (200 if steps >= 1000 else 0) if checkpoint < 0 else checkpoint
+    ii, jj = transducers_ring(n, n, shots, 0.39 * n)
+    e = Engine(cfg, device=device)
+    e.set_phantoms(len(seeds))
+    e.set_transducers(ii, jj, one_based=False)
+    e.set_receivers(ii, jj, one_based=False)
+    e.set_groups(1)
+    e.enable_coverage(True)
+    truth = np.concatenate([phantom(n, int(s), kind) for s in seeds])
+    e.set_truth(truth)
+    e.set_field(truth)
+    meas = e.forward(fp64=True)
+    e.set_field(np.zeros(truth.size, np.float32))
+    e.set_measured(meas)
+    return e
+
+
+def _plain_directions(e: Engine, count: int) -> np.ndarray:
+    """mask * sum_g G_g / (1 + f) / S per phantom of a stacked engine, [count][n2]."""
+    c = e.cfg
+    S = e.num_shots // count
+    f = e.get_field().reshape(count, -1).astype(np.float64)
+    m = np.zeros((c.ny, c.nx))
+    m[c.mask_y0:c.mask_y1, c.mask_x0:c.mask_x1] = 1.0
+    out = np.zeros_like(f)
+    for p in range(count):
+        g = np.zeros(f.shape[1])
+        for s in range(p * S, (p + 1) * S):
+            g += e.get_gradient(s)
+        out[p] = g / S / (1.0 + f[p]) * m.ravel()
+    return out.astype(np.float32)
+
+
+def calibrate_scale_stacked(e: Engine, count: int, frac: float = 0.25, first_max: float = 0.01,
+                            tries: int = 12) -> float:
+    """calibrate_scale for a stack: one step for all its phantoms, the one
+    that minimises their summed data misfit along their first plain
+    directions."""
+    f0 = e.get_field()
+    m_best = e.gradient()
+    d = _plain_directions(e, count).ravel()
+    base = first_max / max(float(np.abs(d).max()), 1e-30)
+    s_best = 0.0
+    for k in range(tries):
+        s = base * 2.0 ** k
+        e.set_field(f0 + s * d)
+        m = e.misfit()
+        if m < m_best:
+            m_best, s_best = m, s
+        else:
+            break
+    e.set_field(f0)
+    if s_best == 0.0:
+        s_best = base
+    return frac * s_best
+
+
+def collect_plain_stacked(net: Preconditioner, e: Engine, count: int, scale: float, iters: int) -> list:
+    """collect_plain (target "best") for every phantom of a stacked engine:
+    at each iterate one sample per phantom, the target the phantom's own
+    best iterate.  Returns the phantoms' best errors."""
+    first = net.num_samples()
+    best_eps = [None] * count
+    best_f = [None] * count
+    for _ in range(iters):
+        e.gradient()
+        for p in range(count):
+            net.add_sample(e, scale, phantom=p)
+        e.update_uniform(scale, UPDATE_MEAN)
+        eps = np.atleast_1d(e.error())
+        f = e.get_field().reshape(count, -1)
+        for p in range(count):
+            if best_eps[p] is None or eps[p] < best_eps[p]:
+                best_eps[p], best_f[p] = float(eps[p]), f[p].copy()
+    for it in range(iters):
+        for p in range(count):
+            net.set_targets(first + it * count + p, 1, best_f[p])
+    return best_eps
+
+
+def train_stacked(net: Preconditioner, n: int, seeds, iters: int, stack: int = 25, steps: int = 0, shots: int = 0,
+                  frac: float = 0.25, train_steps: int = 600, batch: int = 8, lr: float = 1e-3, kind: str = "smooth",
+                  fp: float = 0.1, log=None) -> TrainReport:
+    """train() with target "best" on plain-SIRT trajectories of many
+    phantoms, `stack` phantoms per engine (one relaxation per stack)."""
+    rep = TrainReport()
+    seeds = [int(s) for s in seeds]
+    net.reserve_samples(net.num_samples() + len(seeds) * iters, n, n)
+    t = time.perf_counter()
+    for i in range(0, len(seeds), stack):
+        chunk = seeds[i:i + stack]
+        e = stacked_problem(n, chunk, steps=steps, shots=shots, fp=fp, kind=kind)
+        scale = calibrate_scale_stacked(e, len(chunk), frac)
+        e.set_field(np.zeros(len(chunk) * n * n, np.float32))
+        best = collect_plain_stacked(net, e, len(chunk), scale, iters)
+        e.close()
+        if log:
+            log(f"plain trajectories of phantoms {chunk[0]}..{chunk[-1]}: relaxation {scale:.4g}, best eps "
+                f"{min(best):.2f}..{max(best):.2f}, {net.num_samples()} samples")
+    t_data = time.perf_counter() - t
+    t = time.perf_counter()
+    rep.losses = list(net.train(train_steps, batch, lr, seed=0))
+    rep.seconds = {"trajectories": t_data, "training": time.perf_counter() - t}
+    rep.phantoms = len(seeds)
+    rep.samples = net.num_samples()
+    if log:
+        log(f"trained on {rep.samples} samples: loss {rep.losses[0]:.4f} -> {np.mean(rep.losses[-50:]):.4f}")
     return rep

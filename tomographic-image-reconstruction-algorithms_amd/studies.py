@@ -113,18 +113,29 @@ def _mask(c):
 
 
 def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: str = "best", rounds: int = 0,
-                log=None) -> dict:
+                iters: int = TRAIN_ITERS, stack: int = 0, log=None) -> dict:
     """Reproduce the shipped network (see the module docstring; `phantoms`
-    seeded phantoms from 7000); writes the parameters (float32, numpy .npy)
-    to `out`.  rounds > 0 adds that many rounds of the learned iteration's
-    own states (target: the truth) after the plain trajectories."""
+    seeded phantoms from 7000, `iters` plain iterations each); writes the
+    parameters (float32, numpy .npy) to `out`.  rounds > 0 adds that many
+    rounds of the learned iteration's own states (target: the truth) after
+    the plain trajectories.  stack > 0 runs the trajectories `stack`
+    phantoms per engine, stacked along the shot axis (target "best", one
+    relaxation per stack)."""
     net = Preconditioner(device=0, seed=1)
     t = time.perf_counter()
-    rep = L.train(net, 256, range(7000, 7000 + phantoms), TRAIN_ITERS, frac=TTT_FRAC, rounds=rounds,
-                  train_steps=train_steps, batch=8, lr=1e-3, target=target, fp=TTT_FP, log=log)
+    seeds = range(7000, 7000 + phantoms)
+    if stack > 0:
+        if target != "best" or rounds:
+            raise ValueError("stacked training takes target 'best' and no rounds")
+        rep = L.train_stacked(net, 256, seeds, iters, stack=stack, frac=TTT_FRAC, train_steps=train_steps, batch=8,
+                              lr=1e-3, fp=TTT_FP, log=log)
+    else:
+        rep = L.train(net, 256, seeds, iters, frac=TTT_FRAC, rounds=rounds, train_steps=train_steps, batch=8,
+                      lr=1e-3, target=target, fp=TTT_FP, log=log)
     np.save(out, net.get_params(), allow_pickle=False)
     net.close()
-    return {"study": "train", "out": out, "phantoms": rep.phantoms, "samples": rep.samples,
+    return {"study": "train", "out": out, "phantoms": rep.phantoms, "iterations": iters, "stack": stack,
+            "samples": rep.samples,
             "loss_first": rep.losses[0], "loss_last50": float(np.mean(rep.losses[-50:])),
             "seconds": {k: round(v, 2) for k, v in rep.seconds.items()}, "total_seconds": round(time.perf_counter() - t, 2)}
 
