@@ -271,12 +271,16 @@ def main():
                     help="--study train targets: best (the trajectory's best iterate), ahead:K (K iterations on) "
                          "or truth (the true field)")
     ap.add_argument("--train-iters", type=int, default=50, help="--study train: plain iterations per phantom")
+    ap.add_argument("--train-first-seed", type=int, default=7000,
+                    help="--study train: first phantom seed (with --weights: continue training those parameters)")
     ap.add_argument("--stack", type=int, default=0,
                     help="--study train: phantoms per engine, stacked along the shot axis (0: one engine each)")
     ap.add_argument("--train-rounds", type=int, default=0,
                     help="--study train: rounds that add the learned iteration's own states (target: the truth) "
                          "and train on")
-    ap.add_argument("--weights", default=None, help="network parameters for --study ttt (default: the shipped ones)")
+    ap.add_argument("--weights", default=None,
+                    help="network parameters for --study ttt (default: the shipped ones); for --study train, the "
+                         "parameters to continue from")
     ap.add_argument("--learned-iters", type=int, default=3,
                     help="--study ttt: learned iterations (with the plain fallback) before plain ones")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
@@ -316,7 +320,7 @@ def main():
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
                                 target=args.train_target, rounds=args.train_rounds, iters=args.train_iters,
-                                stack=args.stack, log=log)
+                                stack=args.stack, first_seed=args.train_first_seed, base=args.weights, log=log)
         elif args.study == "finetune":
             out = S.finetune_study(args.out, base=args.weights, phantoms=args.phantoms,
                                    train_steps=args.train_steps, log=log)

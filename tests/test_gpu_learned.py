@@ -74,3 +74,43 @@ def test_shipped_network_gains_over_plain_sirt():
         assert eps_l[-1] < plain.eps[-1] - 5.0
     finally:
         p.close()
+
+
+def test_stacked_samples_equal_single_context_samples():
+    """Samples taken from a stack of phantoms (one batched gradient per
+    iteration, one sample per phantom) are those of one context per phantom:
+    two networks trained on them from the same initial parameters and batch
+    order end bitwise equal."""
+    print("[case] stacked training samples 128^2", flush=True)
+    n, steps, shots, iters, seeds = 128, 256, 8, 3, [7301, 7302]
+    stack = L.stacked_problem(n, seeds, steps=steps, shots=shots)
+    singles = [L.Problem(n, L.phantom(n, s), steps=steps, shots=shots) for s in seeds]
+    try:
+        scale = L.calibrate_scale_stacked(stack, len(seeds), 0.25)
+        stack.set_field(np.zeros(len(seeds) * n * n, np.float32))
+        with tomo.Preconditioner(device=0, seed=9) as a, tomo.Preconditioner(device=0, seed=9) as b:
+            a.reserve_samples(len(seeds) * iters, n, n)
+            best = L.collect_plain_stacked(a, stack, len(seeds), scale, iters)
+            best_f = [None] * len(seeds)
+            best_e = [None] * len(seeds)
+            for _ in range(iters):
+                for p, q in enumerate(singles):
+                    q.engine.gradient()
+                    b.add_sample(q.engine, scale)
+                for p, q in enumerate(singles):
+                    q.engine.update_uniform(scale, tomo.UPDATE_MEAN)
+                    e = q.engine.error()
+                    if best_e[p] is None or e < best_e[p]:
+                        best_e[p], best_f[p] = e, q.engine.get_field()
+            for it in range(iters):
+                for p in range(len(seeds)):
+                    b.set_targets(it * len(seeds) + p, 1, best_f[p])
+            assert best == best_e
+            assert a.num_samples() == b.num_samples() == len(seeds) * iters
+            la, lb = a.train(20, 4, 1e-3, seed=2), b.train(20, 4, 1e-3, seed=2)
+            np.testing.assert_array_equal(la, lb)
+            np.testing.assert_array_equal(a.get_params(), b.get_params())
+    finally:
+        stack.close()
+        for q in singles:
+            q.close()

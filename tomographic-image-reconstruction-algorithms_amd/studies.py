@@ -113,17 +113,21 @@ def _mask(c):
 
 
 def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: str = "best", rounds: int = 0,
-                iters: int = TRAIN_ITERS, stack: int = 0, log=None) -> dict:
+                iters: int = TRAIN_ITERS, stack: int = 0, first_seed: int = 7000, base: str | None = None,
+                log=None) -> dict:
     """Reproduce the shipped network (see the module docstring; `phantoms`
     seeded phantoms from 7000, `iters` plain iterations each); writes the
     parameters (float32, numpy .npy) to `out`.  rounds > 0 adds that many
     rounds of the learned iteration's own states (target: the truth) after
     the plain trajectories.  stack > 0 runs the trajectories `stack`
     phantoms per engine, stacked along the shot axis (target "best", one
-    relaxation per stack)."""
+    relaxation per stack).  base: parameters to continue from (training in
+    phases over consecutive seed ranges, first_seed on)."""
     net = Preconditioner(device=0, seed=1)
+    if base:
+        net.set_params(np.load(base, allow_pickle=False))
     t = time.perf_counter()
-    seeds = range(7000, 7000 + phantoms)
+    seeds = range(first_seed, first_seed + phantoms)
     if stack > 0:
         if target != "best" or rounds:
             raise ValueError("stacked training takes target 'best' and no rounds")
@@ -134,7 +138,8 @@ def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: s
                       lr=1e-3, target=target, fp=TTT_FP, log=log)
     np.save(out, net.get_params(), allow_pickle=False)
     net.close()
-    return {"study": "train", "out": out, "phantoms": rep.phantoms, "iterations": iters, "stack": stack,
+    return {"study": "train", "out": out, "base": base, "first_seed": first_seed, "phantoms": rep.phantoms,
+            "iterations": iters, "stack": stack,
             "samples": rep.samples,
             "loss_first": rep.losses[0], "loss_last50": float(np.mean(rep.losses[-50:])),
             "seconds": {k: round(v, 2) for k, v in rep.seconds.items()}, "total_seconds": round(time.perf_counter() - t, 2)}
