@@ -22,5 +22,6 @@ def test_more_than_one_gpu_is_refused():
 def test_options_parse():
     r = run("--help")
     assert r.returncode == 0
-    for opt in ("--steps", "--warmup", "--time-blocking", "--workload", "--c4-threads", "--study", "--counter-pass"):
+    for opt in ("--steps", "--warmup", "--time-blocking", "--workload", "--c4-threads", "--study", "--counter-pass",
+                "--eager", "--c4-mode", "--stack", "--train-iters", "--train-first-seed", "--fp"):
         assert opt in r.stdout

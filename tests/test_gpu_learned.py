@@ -1,8 +1,9 @@
 """The learned preconditioner end to end on the GPU: trajectory sampling and
 training reduce the loss; the learned iteration never keeps a step that
-raises the data misfit; and the shipped network (weights/preconditioner.npy,
+raises the data misfit; the shipped network (weights/preconditioner.npy,
 trained on 256^2 plain-SIRT trajectories) gains over plain SIRT on a
-held-out phantom."""
+held-out phantom; and training samples taken from stacked phantoms equal
+those of one context per phantom."""
 import os
 
 import numpy as np

@@ -28,12 +28,20 @@
   50 iterations in all.  A seed whose plain run never gets below 100 % is
   flagged and left out of the medians.  Times are wall seconds of each
   method's loop.
-- ``train``: the training run that produced ``weights/preconditioner.npy``:
-  48 seeded 256^2 phantoms (seeds 7000-7047; 64 transducers, 2048 steps,
-  checkpoint 200, source peak frequency 0.05), 50 plain-SIRT iterations each
-  at relaxation 0.3 x the misfit-optimal first step, every iterate a sample
-  whose target is the trajectory's best iterate (lowest error), 4000 Adam
-  steps of batch 8 at learning rate 1e-3.
+- ``train``: the training that produced ``weights/preconditioner.npy``:
+  2000 seeded 256^2 phantoms (seeds 7000-8999; 64 transducers, 2048 steps,
+  checkpoint 200, source peak frequency 0.05), 20 plain-SIRT iterations
+  each, every iterate a sample whose target is the trajectory's best
+  iterate (lowest error).  The trajectories run 25 phantoms per engine,
+  stacked along the shot axis, with one relaxation per stack (0.3 x the
+  misfit-optimal first step of the stack), and the training runs in three
+  phases of 667, 667 and 666 phantoms, each continuing from the previous
+  phase's parameters with 8000 Adam steps of batch 8 at learning rate 1e-3
+  on its own 13,340 samples (``--stack 25 --train-iters 20 --train-steps
+  8000 --phantoms 667 --train-first-seed 7000``, then 7667 and 8334 with
+  ``--weights`` of the previous phase).  ``weights/preconditioner_48x50.npy``
+  is the earlier network: 48 phantoms (seeds 7000-7047) x 50 iterations,
+  one engine each, 4000 Adam steps (``--phantoms 48``, the defaults).
 - ``relax``: how fast plain SIRT itself can go on one held-out ttt phantom:
   50 iterations each of the uniform update at 1, 2 and 3 x the ttt
   relaxation, and at 1 x with per-pixel coverage (illumination) weights of
