@@ -141,3 +141,67 @@ def test_stacked_uniform_iteration_and_refusals():
             e.close()
         if st is not None:
             st.close()
+
+
+def test_stacked_reference_preset_and_passes_across_phantoms():
+    """The reference preset (sensor lines, one-way edges, the scalar and
+    one-step kernels) stacked, with passes of 5 shots that cut across the
+    phantoms' 24 shots (groups of 8 transducers): each phantom's gradients,
+    misfits and uniform update equal a context of its own, bitwise."""
+    from _cases import ref_config, ref_geometry
+    print("[case] stacked phantoms, reference preset, passes across phantoms", flush=True)
+    n, nt, G = 64, 201, 8
+    c = ref_config(n, nt)
+    c.shots_per_pass = 5
+    ii, jj, _ = ref_geometry(c, G)
+    rng = np.random.default_rng(12)
+    fos = []
+    for p in range(3):
+        fo = np.zeros((n, n), np.float32)
+        yy, xx = np.mgrid[0:n, 0:n]
+        cx, cy = rng.uniform(0.4 * n, 0.6 * n, 2)
+        fo[(xx - cx) ** 2 + (yy - cy) ** 2 <= (0.1 * n) ** 2] = rng.uniform(0.02, 0.06)
+        fos.append(fo.ravel())
+    alone = []
+    st = tomo.Engine(c, device=0)
+    try:
+        meas = []
+        for fo in fos:
+            e = tomo.Engine(c, device=0)
+            e.set_transducers(ii, jj, one_based=True)
+            e.set_groups(G)
+            e.set_truth(fo)
+            e.set_field(fo)
+            meas.append(e.forward(fp64=True))
+            e.set_measured(meas[-1])
+            e.set_field(np.zeros(n * n, np.float32))
+            alone.append(e)
+        st.set_phantoms(len(fos))
+        st.set_transducers(ii, jj, one_based=True)
+        S = alone[0].num_shots
+        assert S == 24
+        assert st.set_groups(G) == len(fos) * S
+        st.set_truth(np.concatenate(fos))
+        st.set_field(np.concatenate(fos))
+        np.testing.assert_array_equal(st.forward(fp64=True), np.concatenate(meas))
+        st.set_measured(np.concatenate(meas))
+        st.set_field(np.zeros(len(fos) * n * n, np.float32))
+        for it in range(2):
+            mis = st.gradient()
+            mis_alone = [e.gradient() for e in alone]
+            assert abs(mis - sum(mis_alone)) <= 1e-12 * sum(mis_alone)
+            for p in range(len(fos)):
+                for g in range(S):
+                    np.testing.assert_array_equal(st.get_gradient(p * S + g), alone[p].get_gradient(g))
+            st.update_uniform(2e4, tomo.UPDATE_SUM)
+            for e in alone:
+                e.update_uniform(2e4, tomo.UPDATE_SUM)
+            f = st.get_field().reshape(len(fos), -1)
+            eps = st.error()
+            for p in range(len(fos)):
+                np.testing.assert_array_equal(f[p], alone[p].get_field())
+                assert eps[p] == alone[p].error()
+    finally:
+        st.close()
+        for e in alone:
+            e.close()
