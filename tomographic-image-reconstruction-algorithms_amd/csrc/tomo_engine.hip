@@ -1230,7 +1230,10 @@ int tomo_set_transducers(tomo_ctx *c, const int *ii, const int *jj, int ns, int 
     c->tx[p] = x;
     c->ty[p] = y;
   }
+  // the groups of the previous transducers are gone (tomo_set_groups again)
   c->nshots = 0;
+  c->groups = 0;
+  c->bbox.clear();
   return TOMO_OK;
 }
 
