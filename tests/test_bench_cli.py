@@ -25,3 +25,31 @@ def test_options_parse():
     for opt in ("--steps", "--warmup", "--time-blocking", "--workload", "--c4-threads", "--study", "--counter-pass",
                 "--eager", "--c4-mode", "--stack", "--train-iters", "--train-first-seed", "--fp"):
         assert opt in r.stdout
+
+
+def _bench_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_roofline_figures_from_the_committed_counter_profile():
+    """The bench line's traffic comes from the committed counter summary of
+    the shape it runs, and its fractions follow from the launch time: 20 B
+    per-unit and 12 B blocked bytes of the three-step adjoint over 128 x 512^2
+    x 3 cell-updates per launch."""
+    b = _bench_module()
+    t_adj = b.load_traffic("adjoint3_kernel", (4, 32, 64, 1, 1), 3)
+    t_fwd = b.load_traffic("forward3_kernel", (4, 32, 64, 1, 1), 3)
+    cells = 128 * 512 * 512 * 3
+    assert t_adj is not None and 12.0 <= t_adj / cells < 12.5
+    assert t_fwd is not None and 6.0 <= t_fwd / cells < 7.0
+    assert b.load_traffic("adjoint3_kernel", (4, 16, 16, 1, 1), 3) is None  # no row for another shape
+    line = b.kernel_line("adjoint3_kernel", 20.0, 12.0, cells, 0.275, 1000.0, t_adj, "every launch")
+    assert abs(line["achieved"] - 20.0 * cells / 0.275e-3 / 1e9) < 0.1
+    assert line["frac"] == line["fractions"]["per_unit"]
+    assert abs(line["fractions"]["blocked_algorithmic"] - 12.0 * cells / 0.275e-3 / 8e12) < 1e-4
+    assert abs(line["fractions"]["counter_measured"] - t_adj / 0.275e-3 / 8e12) < 1e-4
+    assert line["bytes_per_cell_update"]["counter_measured"] == round(t_adj / cells, 3)
