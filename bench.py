@@ -602,7 +602,8 @@ def run_c4_stacked(args):
     holding that side's phantoms along the shot axis, so each stencil launch
     advances every shot of every phantom of the side (in passes that keep
     the wavefield storage within 160 GB).  A step is one weighted SIRT
-    iteration of every phantom, each phantom weighted over its own shots.
+    iteration of every phantom, each phantom weighted over its own shots at
+    its own relaxation.
     The engines are dealt by cost (n^4 x phantoms) to --c4-threads host
     threads (default 2), each stepping its engines one after the other on
     their own streams with one shared wavefield workspace; each gradient
@@ -631,18 +632,19 @@ def run_c4_stacked(args):
         if not args.eager:
             e.set_graphs(True)
     log(f"data for {sum(len(x[2]) for x in stacks)} phantoms in {t_data:.1f} s; shapes per side {shapes}")
-    # one relaxation per side: 1e-3 over the largest mean-gradient entry of its phantoms
+    # each phantom's own relaxation, as in a run of its own: 1e-3 over its
+    # largest mean-gradient entry (tomo_update_weighted_each)
     scales = []
     for n, e, idx in stacks:
         e.gradient()
         S = e.num_shots // len(idx)
-        gmax = 0.0
+        sc = np.zeros(len(idx), np.float32)
         for p in range(len(idx)):
             g = np.zeros(n * n, np.float64)
             for s_ in range(p * S, (p + 1) * S):
                 g += e.get_gradient(s_)
-            gmax = max(gmax, float(np.abs(g / S).max()))
-        scales.append(float(1e-3 / max(gmax, 1e-30)))
+            sc[p] = 1e-3 / max(float(np.abs(g / S).max()), 1e-30)
+        scales.append(sc)
 
     def step_group(idx):
         for i in idx:

@@ -281,6 +281,10 @@ int tomo_set_pixel_weights(tomo_ctx *ctx, int mode, double eps);
 
 /* f += scale * mask * (sum_g G_g) / (1 + f) [/ Ng]  (enum tomo_uniform_mode). */
 int tomo_update_uniform(tomo_ctx *ctx, float scale, int mode);
+/* the same with phantom p of a stack (tomo_set_phantoms) at relaxation
+ * scales[p], float[count]: each phantom keeps the relaxation a run of its
+ * own would use.  No reference counterpart (one phantom per process). */
+int tomo_update_uniform_each(tomo_ctx *ctx, const float *scales, int mode);
 
 /* Non-uniform weighted SIRT (sirt-mimo-norm.cu:56-118, 873-892):
  * df_m = G_m / (1 + f); d_m = || df_m - mean_m df_m ||_2 over the full grid;
@@ -288,6 +292,9 @@ int tomo_update_uniform(tomo_ctx *ctx, float scale, int mode);
  * If some d_m is zero the weights fall back to uniform 1/M.
  * weights_out (may be NULL) receives the normalised weights, double[Ng]. */
 int tomo_update_weighted(tomo_ctx *ctx, float scale, double *weights_out);
+/* the same with phantom p of a stack at relaxation scales[p], float[count].
+ * No reference counterpart. */
+int tomo_update_weighted_each(tomo_ctx *ctx, const float *scales, double *weights_out);
 
 /* The update of tomo_update_weighted on caller-owned DEVICE arrays, with no
  * device allocation: no counterpart in the reference; for callers that

@@ -131,6 +131,22 @@ def test_stacked_uniform_iteration_and_refusals():
             _, ep = alone[p].sirt_iteration(5e3, weighted=False, mode=tomo.UPDATE_MEAN)
             assert eps[p] == ep
             np.testing.assert_array_equal(st.get_field().reshape(2, -1)[p], alone[p].get_field())
+        # each phantom at its own relaxation (tomo_update_*_each)
+        scales = np.array([3e3, 7e3], np.float32)
+        st.gradient()
+        st.update_uniform(scales, tomo.UPDATE_MEAN)
+        st.gradient()
+        w = st.update_weighted(scales)
+        for p in range(2):
+            alone[p].gradient()
+            alone[p].update_uniform(float(scales[p]), tomo.UPDATE_MEAN)
+            alone[p].gradient()
+            wp = alone[p].update_weighted(float(scales[p]))
+            S = alone[p].num_shots
+            np.testing.assert_array_equal(w[p * S:(p + 1) * S], wp)
+            np.testing.assert_array_equal(st.get_field().reshape(2, -1)[p], alone[p].get_field())
+        with pytest.raises(ValueError):
+            st.update_uniform(np.ones(3, np.float32))
         st.gradient()
         for call in (lambda: st.update_group(0, 1.0), lambda: st.art_iteration(1.0),
                      lambda: st.line_search(False, 0.0, 100.0, 10)):

@@ -115,6 +115,8 @@ SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "tomo_update_uniform": (_int, [_vp, ctypes.c_float, _int]),
     "tomo_update_weighted": (_int, [_vp, ctypes.c_float, _vp]),
+    "tomo_update_uniform_each": (_int, [_vp, _f32p, _int]),
+    "tomo_update_weighted_each": (_int, [_vp, _f32p, _vp]),
     # device pointers (d_f, d_grad, d_pixw, d_weights) and the stream as plain addresses
     "tomo_update_weighted_noalloc": (_int, [_vp, _vp, _int, _int, _int, _i32p, ctypes.c_double, _vp, _vp, _vp]),
     "tomo_update_avg_weight": (_int, [_vp, ctypes.c_float, _vp]),
@@ -422,12 +424,26 @@ class Engine:
         self._c(lib().tomo_set_pixel_weights(self._ctx, int(mode), float(eps)))
 
     def update_uniform(self, scale, mode=UPDATE_MEAN):
-        self._c(lib().tomo_update_uniform(self._ctx, float(scale), int(mode)))
+        """scale: one relaxation, or one per phantom of a stack."""
+        if np.ndim(scale):
+            self._c(lib().tomo_update_uniform_each(self._ctx, self._scales(scale), int(mode)))
+        else:
+            self._c(lib().tomo_update_uniform(self._ctx, float(scale), int(mode)))
 
     def update_weighted(self, scale):
+        """scale: one relaxation, or one per phantom of a stack; returns the weights."""
         w = np.zeros(self.num_shots, np.float64)
-        self._c(lib().tomo_update_weighted(self._ctx, float(scale), _ptr(w)))
+        if np.ndim(scale):
+            self._c(lib().tomo_update_weighted_each(self._ctx, self._scales(scale), _ptr(w)))
+        else:
+            self._c(lib().tomo_update_weighted(self._ctx, float(scale), _ptr(w)))
         return w
+
+    def _scales(self, scale):
+        s = np.ascontiguousarray(scale, np.float32).ravel()
+        if s.size != self.phantoms:
+            raise ValueError(f"{s.size} scales for {self.phantoms} phantoms")
+        return s
 
     def update_avg_weight(self, scale):
         """Weighted average of intermediate fields (sirt-mimo-avg-weight.cu:

@@ -1476,7 +1476,11 @@ int tomo_get_coverage(tomo_ctx *c, float *out) {
 }
 
 
-int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
+namespace {
+
+// the uniform update of every phantom, phantom p at relaxation scales[p]
+// (scales == nullptr: `scale` for all)
+int update_uniform_phantoms(tomo_ctx *c, float scale, const float *scales, int mode) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
@@ -1487,7 +1491,7 @@ int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
     const float *pw = nullptr;
     if ((rc = pixel_weights(c, p * G, G, &pw))) return rc;
     HIPCHK(c, launch_update_uniform(c->d_f + (size_t)p * c->n2, c->d_grad + (size_t)p * G * c->n2, G, c->nx, c->ny,
-                                    ctx_mask(c), (double)scale, div, c->stream, pw));
+                                    ctx_mask(c), (double)(scales ? scales[p] : scale), div, c->stream, pw));
   }
   if ((rc = upload_coefficients(c))) return rc;
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
@@ -1495,7 +1499,9 @@ int tomo_update_uniform(tomo_ctx *c, float scale, int mode) {
   return TOMO_OK;
 }
 
-int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
+// the weighted update of every phantom, phantom p at relaxation scales[p]
+// (scales == nullptr: `scale` for all)
+int update_weighted_phantoms(tomo_ctx *c, float scale, const float *scales, double *weights_out) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if ((rc = need_shots(c)) || (rc = need_all_gradients(c))) return rc;
@@ -1519,7 +1525,7 @@ int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
       pw = c->d_pixw;
     }
     HIPCHK(c, launch_weighted(c->d_f + (size_t)p * c->n2, c->d_grad + (size_t)p * M * c->n2, M, c->nx, c->ny,
-                              ctx_mask(c), (double)scale, mean, part, pb, w, c->stream, pw));
+                              ctx_mask(c), (double)(scales ? scales[p] : scale), mean, part, pb, w, c->stream, pw));
     if (weights_out)
       HIPCHK(c, hipMemcpyAsync(weights_out + (size_t)p * M, w, sizeof(double) * M, hipMemcpyDeviceToHost,
                                c->stream));
@@ -1528,6 +1534,24 @@ int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
   std::fill(c->grad_valid.begin(), c->grad_valid.end(), 0);  // the field changed
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TOMO_OK;
+}
+
+}  // namespace
+
+int tomo_update_uniform(tomo_ctx *c, float scale, int mode) { return update_uniform_phantoms(c, scale, nullptr, mode); }
+
+int tomo_update_uniform_each(tomo_ctx *c, const float *scales, int mode) {
+  if (!scales) return fail(c, TOMO_ERR_ARG, "null scales");
+  return update_uniform_phantoms(c, 0.f, scales, mode);
+}
+
+int tomo_update_weighted(tomo_ctx *c, float scale, double *weights_out) {
+  return update_weighted_phantoms(c, scale, nullptr, weights_out);
+}
+
+int tomo_update_weighted_each(tomo_ctx *c, const float *scales, double *weights_out) {
+  if (!scales) return fail(c, TOMO_ERR_ARG, "null scales");
+  return update_weighted_phantoms(c, 0.f, scales, weights_out);
 }
 
 int tomo_update_avg_weight(tomo_ctx *c, float scale, double *weights_out) {
