@@ -248,7 +248,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "ref"],
+                    help="c1-c4: the synthetic configurations; ref: the reference's own preset and Table 2 case "
+                         "(192^2, 401 steps, 640 transducers in 64 groups, full history)")
     ap.add_argument("--c4-mode", default="stacked", choices=["stacked", "contexts"],
                     help="C4: one engine per side with that side's phantoms stacked along the shot axis (one launch "
                          "per level advances all of them), or one engine per phantom")
@@ -344,6 +346,9 @@ def main():
                              "median_speedup": float(np.median([x if x is not None else 0.0 for x in sp]))
                              if sp else None}
         print(json.dumps(out), flush=True)
+        return
+    if args.workload == "ref":
+        run_ref(args)
         return
     if args.workload == "c4":
         if args.c4_mode == "stacked":
@@ -594,6 +599,93 @@ def run_c4(args):
         e.close()
     for ws in workspaces:
         ws.close()
+    print(json.dumps(out), flush=True)
+
+
+# The reference's only published timings (report.tex:187-199, Table 2): average
+# SIRT execution time on a Tesla K20c, M = 64 and 32 sensor groups
+REF_TABLE2_S = {64: 0.7625, 32: 0.23898}
+
+
+def run_ref(args):
+    """The reference's own case: its preset (sirt-mimo.cu:17-35: 192^2
+    grid, 401 steps, sensor lines, one-way edges, the reference pulse), its
+    640-transducer square ring in 64 groups of 10 (M = 64 of report.tex
+    Table 2) and its phantom, full wavefield history as the reference keeps
+    it; a step is one uniform SIRT iteration (sirt-mimo-io.cu:326-394).  The
+    report does not say whether Table 2's times are per iteration or per run;
+    the line gives the ratio per iteration."""
+    c = tomo.config_reference()
+    c.checkpoint_every = 0
+    e = tomo.Engine(c, device=0)
+    if not args.eager:
+        e.set_graphs(True)
+    ii, jj = tomo.transducers_reference(640)
+    e.set_transducers(ii, jj, one_based=True)
+    groups = e.set_groups(10)
+    fo = tomo.phantom_reference()
+    e.set_truth(fo)
+    e.set_field(fo)
+    meas = e.forward(fp64=True)
+    e.set_field(np.zeros(fo.size, np.float32))
+    e.set_measured(meas)
+    e.gradient()
+    g = np.stack([e.get_gradient(s) for s in range(groups)]).astype(np.float64).sum(0)
+    scale = float(1e-3 / max(np.abs(g).max(), 1e-30))
+    for _ in range(args.warmup):
+        e.sirt_iteration(scale, weighted=False, mode=tomo.UPDATE_SUM)
+    e.timing_reset()
+    t0 = time.perf_counter()
+    eps = None
+    for _ in range(args.steps):
+        _, eps = e.sirt_iteration(scale, weighted=False, mode=tomo.UPDATE_SUM)
+    elapsed = time.perf_counter() - t0
+    tm = e.timing()
+    cells = tm["fwd_cells"] + tm["adj_cells"]
+    e.close()
+    sec = elapsed / args.steps
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's FP32 gradient of every group on this host's threads
+        # (an iteration less its update)
+        import oracle as orc
+        orc.use_library(native_oracle())
+        orc.set_threads(args.cpu_threads or orc.num_threads())
+        rc, ra = orc.receivers_lines(c)
+        bbox = orc.groups(ii, jj, 10)
+        m32 = np.ascontiguousarray(meas, np.float32)
+        t = time.perf_counter()
+        orc.gradient(c, np.zeros(fo.size, np.float32), bbox, rc, ra, m32, np.float32)
+        t_cpu = time.perf_counter() - t
+        cpu = {"value": round(cells / args.steps / t_cpu / 1e9, 4), "unit": "Gcell-updates/s",
+               "cores": orc.num_threads(), "kind": "port",
+               "build": "oracle/tomo_oracle.cpp, g++ -O3 -march=native -ffp-contract=fast -fopenmp, built on this host",
+               "sample": f"oracle FP32 gradient of all {groups} groups, {c.nx}x{c.ny}, {c.nt} steps, full history: "
+                         f"{t_cpu:.3f} s (the reference's Table 2 CPU time at M = 64: 46.87 s on 8 Xeon cores)",
+               "cpu_model": cpu_model()}
+    out = {
+        "metric": "Gcell-updates/s of FDTD step",
+        "value": round(cells / elapsed / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "the reference's phantom; measured traces from the engine's FP64 solver",
+        "config": {
+            "workload": f"reference preset: {c.nx}x{c.ny} grid, {c.nt} steps, {groups} groups of 10 of the 640 "
+                        "reference transducers, sensor lines, full wavefield history, one uniform SIRT iteration "
+                        "per step",
+            "launch": "eager" if args.eager else "graph replay of every gradient pass",
+            "cell_updates_per_step": cells / args.steps, "eps_after_steps": eps,
+            # Table 2 of the reference's report: K20c seconds per SIRT run or
+            # iteration (not stated) at M = 64; ratio to this engine's iteration
+            "reference_table2": {"k20c_avg_sirt_seconds": REF_TABLE2_S[groups] if groups in REF_TABLE2_S else None,
+                                 "ratio_to_one_iteration": round(REF_TABLE2_S[groups] / sec, 1)
+                                 if groups in REF_TABLE2_S else None},
+        },
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
     print(json.dumps(out), flush=True)
 
 
