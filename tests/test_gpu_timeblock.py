@@ -180,3 +180,43 @@ def test_fallback_to_one_step_adjoint():
         finally:
             e1.close()
             e2.close()
+
+
+@pytest.mark.parametrize("steps", [2, 3])
+def test_reference_preset_blocked_forward_matches_single_steps(steps):
+    """The reference preset (no sponge, sensor lines with a receiver in
+    every cell, records from the small passes after each launch) runs the
+    blocked forward kernels; its traces and gradients equal one step per
+    launch bitwise (the adjoint stays at one step: overwrite injection on
+    the lines)."""
+    from _cases import ref_config, ref_geometry, ref_phantom
+    print(f"[case] reference preset, forward steps per launch {steps}", flush=True)
+    c = ref_config(192, 401)
+    ii, jj, _ = ref_geometry(c, 32)
+    fo = ref_phantom(c)
+    es = []
+    try:
+        for tb in (1, steps):
+            e = tomo.Engine(c, device=0)
+            e.set_transducers(ii, jj, one_based=True)
+            e.set_groups(32)
+            e.set_time_blocking(tb)
+            es.append(e)
+        e1, e2 = es
+        assert e1.time_blocking() == 1 and e2.time_blocking() == steps
+        assert e2.adjoint_blocking() == 1
+        e1.set_field(fo)
+        e2.set_field(fo)
+        meas = e1.forward(fp64=True)
+        np.testing.assert_array_equal(e2.forward(fp64=False), e1.forward(fp64=False))
+        f = (0.5 * fo).astype(np.float32)
+        for e in es:
+            e.set_measured(meas)
+            e.set_field(f)
+        m1, m2 = e1.gradient(), e2.gradient()
+        assert m1 == m2
+        for s in range(e1.num_shots):
+            np.testing.assert_array_equal(e2.get_gradient(s), e1.get_gradient(s))
+    finally:
+        for e in es:
+            e.close()

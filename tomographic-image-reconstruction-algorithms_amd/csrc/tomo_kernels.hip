@@ -2603,8 +2603,11 @@ hipError_t launch_weights_only(const float *f, const float *G, int M, int nx, in
   return hipGetLastError();
 }
 
+// the blocked forward kernels: full-row waves (nx = 64 VEC); the sponge may
+// be absent (the reference preset: a = ab = 1 everywhere, the same FMA chain
+// as the one-step kernels without sponge)
 bool forward2_supported(int nx, int sponge_w) {
-  return sponge_w > 0 && (nx == 128 || nx == 192 || nx == 256 || nx == 384 || nx == 512);
+  return sponge_w >= 0 && sponge_w <= kSpongeLds && (nx == 128 || nx == 192 || nx == 256 || nx == 384 || nx == 512);
 }
 
 hipError_t launch_forward2(Fwd2Args a, int nshots, int rows, int depth, hipStream_t st) {
@@ -2689,7 +2692,9 @@ hipError_t launch_forward2_edges(const Fwd2EdgeArgs &a, int nshots, hipStream_t 
   return hipGetLastError();
 }
 
-bool adjoint2_supported(int nx, int sponge_w) { return forward2_supported(nx, sponge_w); }
+// the blocked adjoint kernels: the sponge preset only (point receivers,
+// additive injection)
+bool adjoint2_supported(int nx, int sponge_w) { return sponge_w > 0 && forward2_supported(nx, sponge_w); }
 
 hipError_t launch_adjoint2(Adj2Args a, int nshots, int rows, hipStream_t st) {
   const int vec = a.nx / kBlockX;
