@@ -388,15 +388,16 @@ int tomo_set_shape(tomo_ctx *ctx, int vec, int fwd_rows, int adj_rows, int fwd_d
 /* Stencil steps per launch in the forward passes: 2 runs two leapfrog
  * steps in one pass over each strip (the intermediate level kept in
  * registers: 8 instead of 12 bytes of traffic per forward cell-update) where
- * it applies -- the sponge preset, vectorised kernels, nx of 128, 192, 256,
- * 384 or 512 (64 lanes of 2, 3, 4, 6 or 8 cells), checkpoint interval >= 4
- * -- and 1 otherwise.  3 (default) runs three
+ * it applies -- vectorised kernels, nx of 128, 192, 256, 384 or 512 (64
+ * lanes of 2, 3, 4, 6 or 8 cells), checkpoint interval >= 4, with or
+ * without the sponge -- and 1 otherwise.  3 (default) runs three
  * forward steps per launch where, in addition, the strips tile the rows
  * exactly and the checkpoint interval is >= 5 (a recording pass then writes
  * only the last two levels of each launch: 16 bytes per three
  * cell-updates), else two or one.  Where the forward pass runs two or three
  * steps, the adjoint sweep runs two (16 instead of 24 bytes per
- * cell-update) when the receivers are points with additive injection and no
+ * cell-update) with the sponge preset, when the receivers are points with
+ * additive injection and no
  * 64th of a row holds two of them; the coverage map (tomo_enable_coverage)
  * is then summed per checkpoint segment by a pass of its own, in the
  * one-step sweep's order.  Results are the same bit for bit.  tomo_get_time_blocking returns the forward steps per
