@@ -221,3 +221,38 @@ def test_stacked_reference_preset_and_passes_across_phantoms():
         st.close()
         for e in alone:
             e.close()
+
+
+def test_phantom_arguments_are_checked():
+    """Bad phantom counts and indices are refused with an error, and a
+    context keeps working after them."""
+    print("[case] stacked phantoms: argument checks", flush=True)
+    n = 128
+    c = config(n)
+    ii, jj = geometry(n)
+    with tomo.Engine(c, device=0) as e:
+        for bad in (0, -1, 70000):
+            with pytest.raises(tomo.TomoError):
+                e.set_phantoms(bad)
+        e.set_phantoms(2)
+        e.set_transducers(ii, jj, one_based=False)
+        e.set_receivers(ii, jj, one_based=False)
+        assert e.set_groups(1) == 2 * len(ii)
+        fos = [unit_phantom(c, seed=80 + p) for p in range(2)]
+        e.set_truth(np.concatenate(fos))
+        e.set_field(np.concatenate(fos))
+        e.set_measured(e.forward(fp64=True))
+        e.set_field(np.zeros(2 * n * n, np.float32))
+        e.gradient()
+        with tomo.Preconditioner(device=0, seed=1) as net:
+            for bad in (-1, 2):
+                with pytest.raises(tomo.TomoError):
+                    net.add_sample(e, 1.0, phantom=bad)
+            with pytest.raises(tomo.TomoError):
+                net.add_sample(e, 1.0)  # the one-phantom call on a stack
+            net.add_sample(e, 1.0, phantom=1)
+            assert net.num_samples() == 1
+        # new transducers drop the groups: the shots must be set up again
+        e.set_transducers(ii, jj, one_based=False)
+        with pytest.raises(tomo.TomoError):
+            e.gradient()
