@@ -70,6 +70,18 @@ StencilShape stencil_shape(const tomo_ctx *c) {
   return s;
 }
 
+// strip height of the blocked forward launches over passes of np shots: the
+// shape's, or 8 rows when the strips would fill less than a quarter of the
+// wave slots (a launch over a few shots, e.g. an ART step's single shot,
+// runs on the latency of each strip's row-by-row walk, which shorter strips
+// cut; fuller launches keep the taller strips and their smaller halo share)
+int forward_rows(const tomo_ctx *c, int np) {
+  const int r = stencil_shape(c).fwd_rows;
+  if (c->shape_req.fwd_rows || r <= 8 || c->ny % 8 != 0) return r;
+  const long long slots = 8LL * c->num_cus;  // two waves per SIMD
+  return 4LL * np * ((c->ny + r - 1) / r) < slots ? 8 : r;
+}
+
 // strip height of the two-step adjoint: the forward's rule (its halo is two
 // rows of z(k+1) per strip, its register window does not grow with it);
 // measured at 512^2 x 128 shots: 32 rows 5 % faster than 16
@@ -531,7 +543,7 @@ int forward_step2(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
     b.level1 = k + 1;
   }
   const StencilShape sh = stencil_shape(c);
-  HIPCHK(c, launch_forward2(b, P, sh.fwd_rows, sh.fwd_depth, c->stream));
+  HIPCHK(c, launch_forward2(b, P, forward_rows(c, P), sh.fwd_depth, c->stream));
   if ((b.rec1 || b.rec2) && !in_kernel) {  // records over the receivers
     Fwd2EdgeArgs e{};
     e.u0 = u0;
@@ -604,7 +616,7 @@ int forward_step3(tomo_ctx *c, FwdArgs<float> a, const float *um, const float *u
     b.nr = c->nr;
     b.level1 = k + 1;
   }
-  HIPCHK(c, launch_forward3(t, P, stencil_shape(c).fwd_rows, c->stream));
+  HIPCHK(c, launch_forward3(t, P, forward_rows(c, P), c->stream));
   if (any && !in_kernel) {  // records over the receivers: levels k+1, k+2, then k+3
     Fwd2EdgeArgs e{};
     e.sstride = a.sstride;
