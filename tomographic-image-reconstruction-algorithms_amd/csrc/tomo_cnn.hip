@@ -746,16 +746,19 @@ int ensure_engine_scratch(tomo_cnn *n, long long n2) {
   return TOMO_OK;
 }
 
-// network input of the context's current state into d_x8 (and the target
-// into `target` when non-null); needs gradients of every shot
-// sample storage for `cap` samples of n2 cells (the stored ones are kept)
+// sample storage for `cap` samples of n2 cells (the stored ones are kept;
+// on failure the old storage stays and nothing new is held)
 int reserve_samples(tomo_cnn *n, int cap, size_t n2) {
   int rc;
   __bf16 *sx = nullptr;
   float *st = nullptr, *sf = nullptr;
   if ((rc = nalloc(n, &sx, (size_t)cap * n2 * 8)) || (rc = nalloc(n, &st, (size_t)cap * n2)) ||
-      (rc = nalloc(n, &sf, (size_t)cap * n2)))
+      (rc = nalloc(n, &sf, (size_t)cap * n2))) {
+    if (sx) (void)hipFree(sx);
+    if (st) (void)hipFree(st);
+    if (sf) (void)hipFree(sf);
     return rc;
+  }
   if (n->nsamples) {
     CNNCHK(n, hipMemcpy(sx, n->d_sx, sizeof(__bf16) * n->nsamples * n2 * 8, hipMemcpyDeviceToDevice));
     CNNCHK(n, hipMemcpy(st, n->d_st, sizeof(float) * n->nsamples * n2, hipMemcpyDeviceToDevice));
@@ -772,8 +775,9 @@ int reserve_samples(tomo_cnn *n, int cap, size_t n2) {
   return TOMO_OK;
 }
 
-// network input of phantom p of the context (its field, its own shots'
-// plain step and coverage, its truth for the target)
+// network input of phantom p of the context's current state into d_x8 (its
+// field, its own shots' plain step and coverage) and, when `target` is
+// non-null, the target from its truth; needs gradients of every shot
 int engine_input(tomo_cnn *n, tomo_ctx *c, int p, float scale, float *target) {
   int rc;
   if ((rc = ensure_engine_scratch(n, c->n2))) return rc;
@@ -983,8 +987,8 @@ int tomo_cnn_add_sample_phantom(tomo_cnn *n, tomo_ctx *c, int p, float scale) {
   if ((rc = engine_input(n, c, p, scale, n->d_st + (size_t)n->nsamples * n2))) return rc;
   CNNCHK(n, hipMemcpyAsync(n->d_sx + (size_t)n->nsamples * n2 * 8, n->d_x8, sizeof(__bf16) * n2 * 8,
                            hipMemcpyDeviceToDevice, c->stream));
-  CNNCHK(n, hipMemcpyAsync(n->d_sf + (size_t)n->nsamples * n2, c->d_f + (size_t)p * n2, sizeof(float) * n2, hipMemcpyDeviceToDevice,
-                           c->stream));
+  CNNCHK(n, hipMemcpyAsync(n->d_sf + (size_t)n->nsamples * n2, c->d_f + (size_t)p * n2, sizeof(float) * n2,
+                           hipMemcpyDeviceToDevice, c->stream));
   CNNCHK(n, hipStreamSynchronize(c->stream));
   n->nsamples += 1;
   return TOMO_OK;
