@@ -1101,7 +1101,10 @@ int stack_shots(tomo_ctx *c) {
   if ((rc = dalloc(c, &c->d_bbox, all.size()))) return rc;
   HIPCHK(c, hipMemcpy(c->d_bbox, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
   c->nshots = (int)all.size();
-  if ((rc = alloc_shot_buffers(c))) return rc;
+  if ((rc = alloc_shot_buffers(c))) {
+    c->nshots = 0;  // no shot may run on the buffers that failed (tomo_set_groups again)
+    return rc;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TOMO_OK;
 }
@@ -1272,12 +1275,25 @@ int tomo_set_phantoms(tomo_ctx *c, int count) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (count < 1 || count > 65536) return fail(c, TOMO_ERR_ARG, "phantom count out of range");
-  if ((double)count * c->n2 > 4e9) return fail(c, TOMO_ERR_ARG, "phantom images exceed 2^32 cells");
+  if ((double)count * c->n2 > 4e9) return fail(c, TOMO_ERR_ARG, "phantom images exceed 4e9 cells");
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  graphs_clear(c);
-  c->nph = count;
+  // the new images first: on an allocation failure the context keeps its own
   const size_t n = (size_t)count * c->n2;
-  if ((rc = dalloc(c, &c->d_f, n)) || (rc = dalloc(c, &c->d_fo, n)) || (rc = dalloc(c, &c->d_pw, n))) return rc;
+  float *f = nullptr, *fo = nullptr, *pw = nullptr;
+  if ((rc = dalloc(c, &f, n)) || (rc = dalloc(c, &fo, n)) || (rc = dalloc(c, &pw, n))) {
+    dfree(f);
+    dfree(fo);
+    dfree(pw);
+    return rc;
+  }
+  graphs_clear(c);
+  dfree(c->d_f);
+  dfree(c->d_fo);
+  dfree(c->d_pw);
+  c->d_f = f;
+  c->d_fo = fo;
+  c->d_pw = pw;
+  c->nph = count;
   dfree(c->d_pw64);
   HIPCHK(c, launch_fill_f32(c->d_f, n, 0.f, c->stream));
   HIPCHK(c, launch_fill_f32(c->d_fo, n, 0.f, c->stream));
