@@ -285,6 +285,11 @@ def main():
                          "parameters to continue from")
     ap.add_argument("--learned-iters", type=int, default=3,
                     help="--study ttt: learned iterations (with the plain fallback) before plain ones")
+    ap.add_argument("--learned-alpha", type=float, default=1.0,
+                    help="--study ttt: the learned iterations' mix, alpha * network step + (1 - alpha) * plain step")
+    ap.add_argument("--momentum", default="",
+                    help="--study relax: heavy-ball momenta to compare (comma separated); --study ttt: the momentum "
+                         "of the learned run's plain iterations, with plain SIRT at the same momentum beside it")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--time-blocking", type=int, default=3, choices=[1, 2, 3],
                     help="forward steps per launch (2, 3: the intermediate levels in registers)")
@@ -318,7 +323,8 @@ def main():
         if args.study == "c2":
             out = S.c2_study(fp=args.fp or 0.1, log=log)
         elif args.study == "relax":
-            out = S.relax_study(seed=int(args.seeds.split(",")[0]), log=log)
+            out = S.relax_study(seed=int(args.seeds.split(",")[0]), log=log,
+                                momenta=[float(b) for b in args.momentum.split(",")] if args.momentum else None)
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
                                 target=args.train_target, rounds=args.train_rounds, iters=args.train_iters,
@@ -329,10 +335,14 @@ def main():
         else:
             runs = []
             for v in args.seeds.split(","):
-                runs.append(S.ttt_study(seed=int(v), weights=args.weights, learned_iters=args.learned_iters, log=log))
+                runs.append(S.ttt_study(seed=int(v), weights=args.weights, learned_iters=args.learned_iters,
+                                        momentum=float(args.momentum or 0.0), alpha=args.learned_alpha, log=log))
                 # one line per seed as it finishes (the whole study runs for minutes)
                 print(json.dumps({"ttt_seed": int(v), **{k: runs[-1][k] for k in (
-                    "plain_never_below_100", "speedup_fixed50", "speedup_stop_rule", "speedup_fixed10")}}),
+                    "plain_never_below_100", "speedup_fixed50", "speedup_stop_rule", "speedup_fixed10")},
+                                  **({"momentum_plain_speedup_fixed50":
+                                      runs[-1]["targets"]["fixed50"]["momentum_plain_speedup"]}
+                                     if runs[-1]["momentum"] else {})}),
                       file=sys.stderr, flush=True)
             out = {"study": "ttt", "runs": runs, "flagged_never_below_100": [r["phantom_seed"] for r in runs
                                                                              if r["plain_never_below_100"]]}
@@ -345,6 +355,13 @@ def main():
                              "at_least_3x": sum(1 for x in reached if x >= 3.0),
                              "median_speedup": float(np.median([x if x is not None else 0.0 for x in sp]))
                              if sp else None}
+                if counted and counted[0]["momentum"]:
+                    # the same momentum without the network
+                    mp = [r["targets"][name]["momentum_plain_speedup"] for r in counted]
+                    out[name]["momentum_plain"] = {
+                        "speedups": mp, "reached": f"{sum(1 for x in mp if x is not None)} of {len(counted)}",
+                        "at_least_3x": sum(1 for x in mp if x is not None and x >= 3.0),
+                        "median_speedup": float(np.median([x if x is not None else 0.0 for x in mp]))}
         print(json.dumps(out), flush=True)
         return
     if args.workload == "ref":

@@ -169,15 +169,27 @@ class Trace:
 
 
 def run_plain(p: Problem, scale: float, iters: int, weighted: bool = False, target: float | None = None,
-              stop_on_rise: bool = False) -> Trace:
+              stop_on_rise: bool = False, momentum: float = 0.0) -> Trace:
     """Plain SIRT for up to `iters` iterations; stops early at `target`, or
     (stop_on_rise) at the first iteration whose error exceeds the previous
-    one, as the reference loop does (sirt-mimo-io.cu:379-393)."""
+    one, as the reference loop does (sirt-mimo-io.cu:379-393).
+
+    momentum > 0 adds the heavy-ball term momentum * (f_k - f_{k-1}) after
+    each update (Polyak 1964), restarted (no term) at every iterate whose data
+    misfit exceeds the previous iterate's (the function-value restart of
+    O'Donoghue and Candes 2015).  It reads only the misfit, never the truth."""
     tr = Trace()
     e = p.engine
     t0 = time.perf_counter()
+    f_prev, m_prev = None, None
     for _ in range(iters):
+        f_k = e.get_field() if momentum else None
         m, eps = e.sirt_iteration(scale, weighted=weighted, mode=UPDATE_MEAN)
+        if momentum:
+            if f_prev is not None and m <= m_prev:
+                e.set_field(e.get_field() + np.float32(momentum) * (f_k - f_prev))
+                eps = e.error()
+            f_prev, m_prev = f_k, m
         rose = stop_on_rise and len(tr.eps) > 0 and eps > tr.eps[-1]
         tr.add(m, eps, True, time.perf_counter() - t0)
         if (target is not None and eps <= target) or rose:

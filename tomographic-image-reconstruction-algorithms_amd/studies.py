@@ -27,7 +27,11 @@
   and continues with plain iterations until it reaches the lower target or
   50 iterations in all.  A seed whose plain run never gets below 100 % is
   flagged and left out of the medians.  Times are wall seconds of each
-  method's loop.
+  method's loop.  With ``--momentum B`` the learned run's plain iterations
+  carry heavy-ball momentum B (``learned.run_plain``), and plain SIRT with
+  the same momentum and no network runs beside it to the same targets;
+  ``--learned-alpha`` sets the learned iterations' mix of network and plain
+  step.
 - ``train``: the training that produced ``weights/preconditioner.npy``:
   2000 seeded 256^2 phantoms (seeds 7000-8999; 64 transducers, 2048 steps,
   checkpoint 200, source peak frequency 0.05), 20 plain-SIRT iterations
@@ -45,7 +49,8 @@
 - ``relax``: how fast plain SIRT itself can go on one held-out ttt phantom:
   50 iterations each of the uniform update at 1, 2 and 3 x the ttt
   relaxation, and at 1 x with per-pixel coverage (illumination) weights of
-  eps 0.05 and 0.2.
+  eps 0.05 and 0.2; with ``--momentum B1,B2,...``, the uniform update at the
+  ttt relaxation with each heavy-ball momentum instead.
 - ``finetune``: continues training a network on plain-SIRT trajectories at
   the evaluation size, 512^2 with 128 transducers and 4000 steps (seeds from
   8000, disjoint from the held-out 9100-9107), targets as in ``train``.
@@ -173,10 +178,14 @@ def finetune_study(out: str, base: str | None = None, phantoms: int = 12, iters:
 
 
 def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = TTT_FRAC, seed: int = 9100,
-              weights: str | None = None, fp: float = TTT_FP, log=None) -> dict:
+              weights: str | None = None, fp: float = TTT_FP, momentum: float = 0.0, alpha: float = 1.0,
+              log=None) -> dict:
     """Plain SIRT runs `plain_iters` iterations and sets the two targets; the
     learned run takes `learned_iters` learned iterations and continues with
-    plain ones until it reaches the lower target (see the module docstring)."""
+    plain ones until it reaches the lower target (see the module docstring).
+    With `momentum`, the learned run's plain iterations carry that heavy-ball
+    momentum, and plain SIRT with the same momentum (no network) runs beside
+    it to the same target, so that the network's share of the gain shows."""
     n = 512
     net = Preconditioner(device=0)
     net.set_params(np.load(weights or WEIGHTS, allow_pickle=False))
@@ -202,15 +211,20 @@ def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = TTT_F
     p.engine.enable_coverage(True)
     t0 = time.perf_counter()
     goal = min(targets["fixed50"], targets["stop_rule"])
-    tl = L.run_learned(net, p, scale, learned_iters, target=goal)
+    tl = L.run_learned(net, p, scale, learned_iters, alpha=alpha, target=goal)
     eps, secs, acc = list(tl.eps), list(tl.seconds), list(tl.accepted)
     if eps[-1] > goal and len(eps) < plain_iters:
         base = time.perf_counter() - t0
         p.engine.enable_coverage(False)
-        tq = L.run_plain(p, scale, plain_iters - len(eps), target=goal)
+        tq = L.run_plain(p, scale, plain_iters - len(eps), target=goal, momentum=momentum)
         eps += tq.eps
         secs += [base + s for s in tq.seconds]
         acc += [True] * len(tq.eps)
+    tm = None
+    if momentum:
+        p.reset()
+        p.engine.enable_coverage(False)
+        tm = L.run_plain(p, scale, plain_iters, target=goal, momentum=momentum)
     p.close()
     net.close()
     res = {}
@@ -222,37 +236,55 @@ def ttt_study(plain_iters: int = 50, learned_iters: int = 3, frac: float = TTT_F
                      "learned_iterations": None if k_l is None else k_l + 1,
                      "learned_seconds": None if t_l is None else round(t_l, 2),
                      "speedup": None if (t_l is None or never_below_100) else round(t_p / t_l, 2)}
+        if tm is not None:
+            k_m = next((i for i, e in enumerate(tm.eps) if e <= tgt), None)
+            t_m = tm.seconds[k_m] if k_m is not None else None
+            res[name]["momentum_plain_iterations"] = None if k_m is None else k_m + 1
+            res[name]["momentum_plain_seconds"] = None if t_m is None else round(t_m, 2)
+            res[name]["momentum_plain_speedup"] = (None if (t_m is None or never_below_100)
+                                                   else round(t_p / t_m, 2))
         if log:
             log(f"target {name} {tgt:.3f}: plain {k_p + 1} iterations {t_p:.1f} s, learned "
-                + (f"{k_l + 1} iterations {t_l:.1f} s ({t_p / t_l:.2f}x)" if t_l is not None else "not reached"))
+                + (f"{k_l + 1} iterations {t_l:.1f} s ({t_p / t_l:.2f}x)" if t_l is not None else "not reached")
+                + ("" if tm is None else f"; plain with momentum {momentum:g}: "
+                   + (f"{res[name]['momentum_plain_iterations']} iterations "
+                      f"({res[name]['momentum_plain_speedup']}x)" if res[name]["momentum_plain_seconds"] is not None
+                      else "not reached")))
     return {
         "study": "ttt", "grid": n, "shots": 128, "time_steps": 4000, "ricker_fp": fp, "phantom_seed": seed,
-        "relaxation": scale, "relaxation_frac": frac, "calibration_seconds": round(t_cal, 2),
+        "relaxation": scale, "relaxation_frac": frac, "calibration_seconds": round(t_cal, 2), "momentum": momentum,
+        "learned_alpha": alpha,
         "plain_never_below_100": never_below_100,
         "plain_stop_iteration": k_stop + 1, "plain_error_rose": k_rise is not None,
         "plain_eps": [round(v, 3) for v in eps_p],
         "learned_eps": [round(v, 3) for v in eps], "learned_steps_accepted": acc[:learned_iters],
+        "momentum_plain_eps": None if tm is None else [round(v, 3) for v in tm.eps],
         "targets": res,
         "speedup_fixed50": res["fixed50"]["speedup"], "speedup_stop_rule": res["stop_rule"]["speedup"],
         "speedup_fixed10": res["fixed10"]["speedup"],
     }
 
 
-def relax_study(seed: int = 9101, iters: int = 50, fp: float = TTT_FP, log=None) -> dict:
+def relax_study(seed: int = 9101, iters: int = 50, fp: float = TTT_FP, momenta=None, log=None) -> dict:
     """Plain SIRT on the ttt problem of `seed` at several relaxations and with
     coverage weights (see the module docstring): the error trajectory of each
-    variant and its wall seconds."""
+    variant and its wall seconds.  With `momenta`, the variants are instead
+    the uniform update at the ttt relaxation with each heavy-ball momentum
+    (``learned.run_plain``)."""
     n = 512
     p = L.Problem(n, L.phantom(n, seed), steps=4000, shots=128, checkpoint=200, fp=fp)
     scale = L.calibrate_scale(p, TTT_FRAC)
     rows = []
-    for name, weights, eps_w, mult in [("uniform x1", 0, 0.05, 1.0), ("coverage eps 0.05", 1, 0.05, 1.0),
-                                       ("coverage eps 0.2", 1, 0.2, 1.0), ("uniform x2", 0, 0.05, 2.0),
-                                       ("uniform x3", 0, 0.05, 3.0)]:
+    variants = [("uniform x1", 0, 0.05, 1.0, 0.0), ("coverage eps 0.05", 1, 0.05, 1.0, 0.0),
+                ("coverage eps 0.2", 1, 0.2, 1.0, 0.0), ("uniform x2", 0, 0.05, 2.0, 0.0),
+                ("uniform x3", 0, 0.05, 3.0, 0.0)]
+    if momenta:
+        variants = [(f"uniform x1, momentum {b:g}", 0, 0.05, 1.0, float(b)) for b in momenta]
+    for name, weights, eps_w, mult, beta in variants:
         p.reset()
         p.engine.enable_coverage(weights == 1)
         p.engine.set_pixel_weights(weights, eps_w)
-        tr = L.run_plain(p, scale * mult, iters)
+        tr = L.run_plain(p, scale * mult, iters, momentum=beta)
         rows.append({"variant": name, "eps": [round(v, 3) for v in tr.eps], "seconds": round(tr.seconds[-1], 2)})
         if log:
             log(f"{name}: eps after 10 / 25 / {iters} iterations {tr.eps[9]:.2f} / {tr.eps[24]:.2f} / "
