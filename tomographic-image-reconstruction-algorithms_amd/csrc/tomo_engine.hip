@@ -1097,12 +1097,15 @@ int stack_shots(tomo_ctx *c) {
   const int ng = c->groups;
   std::vector<int4> all((size_t)ng * c->nph);
   for (int p = 0; p < c->nph; ++p) std::copy(c->bbox.begin(), c->bbox.end(), all.begin() + (size_t)p * ng);
+  // no shots until every per-shot buffer is in place: a failure below
+  // leaves none to run (tomo_set_groups again)
+  c->nshots = 0;
   int rc;
   if ((rc = dalloc(c, &c->d_bbox, all.size()))) return rc;
   HIPCHK(c, hipMemcpy(c->d_bbox, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
   c->nshots = (int)all.size();
   if ((rc = alloc_shot_buffers(c))) {
-    c->nshots = 0;  // no shot may run on the buffers that failed (tomo_set_groups again)
+    c->nshots = 0;
     return rc;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
