@@ -289,7 +289,8 @@ def main():
                     help="--study ttt: the learned iterations' mix, alpha * network step + (1 - alpha) * plain step")
     ap.add_argument("--momentum", default="",
                     help="--study relax: heavy-ball momenta to compare (comma separated); --study ttt: the momentum "
-                         "of the learned run's plain iterations, with plain SIRT at the same momentum beside it")
+                         "of the learned run's plain iterations, with plain SIRT at the same momentum beside it; "
+                         "--study train (with --stack): the momentum of the training trajectories")
     ap.add_argument("--autotune", action="store_true", help="choose the stencil shape with the autotuner")
     ap.add_argument("--time-blocking", type=int, default=3, choices=[1, 2, 3],
                     help="forward steps per launch (2, 3: the intermediate levels in registers)")
@@ -328,7 +329,8 @@ def main():
         elif args.study == "train":
             out = S.train_study(args.out, phantoms=args.phantoms, train_steps=args.train_steps,
                                 target=args.train_target, rounds=args.train_rounds, iters=args.train_iters,
-                                stack=args.stack, first_seed=args.train_first_seed, base=args.weights, log=log)
+                                stack=args.stack, first_seed=args.train_first_seed, base=args.weights,
+                                momentum=float(args.momentum or 0.0), log=log)
         elif args.study == "finetune":
             out = S.finetune_study(args.out, base=args.weights, phantoms=args.phantoms,
                                    train_steps=args.train_steps, log=log)

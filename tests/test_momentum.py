@@ -102,3 +102,68 @@ def test_momentum_on_a_slow_mode_follows_the_recurrence():
     A = e0 - B
     assert heavy.eps[-1] == pytest.approx(100 * abs(A * r1 ** k + B * r2 ** k), rel=1e-4)
     assert heavy.eps[-1] < 0.3 * plain.eps[-1]
+
+
+class Stack:
+    """Stand-in stacked engine: phantom p is Quadratic(lam, targets[p]) with
+    one common lam, so each phantom's error and misfit rise and fall
+    together and both restart rules agree.  Records every field error()
+    sees."""
+
+    def __init__(self, lam, targets):
+        self.lam = float(lam)
+        self.t = np.asarray(targets, np.float64)
+        self.f = np.zeros_like(self.t)
+        self.seen = []
+
+    def gradient(self):
+        self.d = -self.lam * (self.f - self.t)
+        return 0.5 * self.lam * float(np.sum((self.f - self.t) ** 2))
+
+    def update_uniform(self, scale, mode=None):
+        self.f = self.f + scale * self.d
+
+    def error(self):
+        self.seen.append(self.f.copy())
+        return 100.0 * np.linalg.norm(self.f - self.t, axis=1) / np.linalg.norm(self.t, axis=1)
+
+    def get_field(self):
+        return self.f.astype(np.float32).ravel()
+
+    def set_field(self, f):
+        self.f = np.asarray(f, np.float64).reshape(self.t.shape)
+
+
+class Samples:
+    def __init__(self):
+        self.n, self.targets = 0, {}
+
+    def num_samples(self):
+        return self.n
+
+    def add_sample(self, e, scale, phantom=0):
+        self.n += 1
+
+    def set_targets(self, first, count, f):
+        for i in range(first, first + count):
+            self.targets[i] = np.asarray(f).copy()
+
+
+def test_stacked_training_trajectories_with_momentum():
+    """collect_plain_stacked with momentum: each phantom's trajectory is the
+    one run_plain takes on that phantom alone (restarts included), and every
+    sample's target is the phantom's best iterate."""
+    targets = [[1.0, 2.0, -1.0], [0.5, -0.25, 3.0]]
+    st, net = Stack(0.9, targets), Samples()
+    L.collect_plain_stacked(net, st, 2, 1.6, 8, momentum=0.7)
+    seen = np.stack(st.seen[1:])  # the first error() is of the starting field
+    for p, t in enumerate(targets):
+        q = Quadratic([0.9] * 3, t)
+        tr = run(q, 1.6, 8, 0.7)
+        alone = np.stack(q.history[1:] + [q.f])
+        np.testing.assert_allclose(seen[:, p], alone, rtol=1e-6, atol=1e-7)
+        best = alone[int(np.argmin(tr.eps))]
+        for k in range(8):
+            np.testing.assert_allclose(net.targets[2 * k + p], best, rtol=1e-6, atol=1e-7)
+    # s lam = 1.44 overshoots: the error rises somewhere, so a restart is exercised
+    assert any(b > a for a, b in zip(tr.eps, tr.eps[1:]))

@@ -379,19 +379,35 @@ def calibrate_scale_stacked(e: Engine, count: int, frac: float = 0.25, first_max
     return frac * s_best
 
 
-def collect_plain_stacked(net: Preconditioner, e: Engine, count: int, scale: float, iters: int) -> list:
+def collect_plain_stacked(net: Preconditioner, e: Engine, count: int, scale: float, iters: int,
+                          momentum: float = 0.0) -> list:
     """collect_plain (target "best") for every phantom of a stacked engine:
     at each iterate one sample per phantom, the target the phantom's own
-    best iterate.  Returns the phantoms' best errors."""
+    best iterate.  Returns the phantoms' best errors.
+
+    momentum > 0 runs the trajectories with heavy-ball momentum (as
+    run_plain), restarted per phantom at an iterate whose image error rose
+    (training phantoms have a known truth; the engine reports no misfit per
+    phantom)."""
     first = net.num_samples()
     best_eps = [None] * count
     best_f = [None] * count
+    # the previous iterate and the errors of it and of the current one
+    f_prev, eps_prev = None, None
+    eps_cur = np.atleast_1d(e.error()) if momentum else None
     for _ in range(iters):
         e.gradient()
         for p in range(count):
             net.add_sample(e, scale, phantom=p)
+        f_k = e.get_field().reshape(count, -1) if momentum else None
         e.update_uniform(scale, UPDATE_MEAN)
+        if momentum and f_prev is not None:
+            keep = (eps_cur <= eps_prev).astype(np.float32)[:, None]
+            f = e.get_field().reshape(count, -1)
+            e.set_field((f + np.float32(momentum) * keep * (f_k - f_prev)).ravel())
         eps = np.atleast_1d(e.error())
+        if momentum:
+            f_prev, eps_prev, eps_cur = f_k, eps_cur, eps
         f = e.get_field().reshape(count, -1)
         for p in range(count):
             if best_eps[p] is None or eps[p] < best_eps[p]:
@@ -404,7 +420,7 @@ def collect_plain_stacked(net: Preconditioner, e: Engine, count: int, scale: flo
 
 def train_stacked(net: Preconditioner, n: int, seeds, iters: int, stack: int = 25, steps: int = 0, shots: int = 0,
                   frac: float = 0.25, train_steps: int = 600, batch: int = 8, lr: float = 1e-3, kind: str = "smooth",
-                  fp: float = 0.1, log=None) -> TrainReport:
+                  fp: float = 0.1, momentum: float = 0.0, log=None) -> TrainReport:
     """train() with target "best" on plain-SIRT trajectories of many
     phantoms, `stack` phantoms per engine (one relaxation per stack)."""
     rep = TrainReport()
@@ -416,7 +432,7 @@ def train_stacked(net: Preconditioner, n: int, seeds, iters: int, stack: int = 2
         e = stacked_problem(n, chunk, steps=steps, shots=shots, fp=fp, kind=kind)
         scale = calibrate_scale_stacked(e, len(chunk), frac)
         e.set_field(np.zeros(len(chunk) * n * n, np.float32))
-        best = collect_plain_stacked(net, e, len(chunk), scale, iters)
+        best = collect_plain_stacked(net, e, len(chunk), scale, iters, momentum=momentum)
         e.close()
         if log:
             log(f"plain trajectories of phantoms {chunk[0]}..{chunk[-1]}: relaxation {scale:.4g}, best eps "

@@ -127,7 +127,7 @@ def _mask(c):
 
 def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: str = "best", rounds: int = 0,
                 iters: int = TRAIN_ITERS, stack: int = 0, first_seed: int = 7000, base: str | None = None,
-                log=None) -> dict:
+                momentum: float = 0.0, log=None) -> dict:
     """Reproduce the shipped network (see the module docstring; `phantoms`
     seeded phantoms from 7000, `iters` plain iterations each); writes the
     parameters (float32, numpy .npy) to `out`.  rounds > 0 adds that many
@@ -135,7 +135,8 @@ def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: s
     the plain trajectories.  stack > 0 runs the trajectories `stack`
     phantoms per engine, stacked along the shot axis (target "best", one
     relaxation per stack).  base: parameters to continue from (training in
-    phases over consecutive seed ranges, first_seed on)."""
+    phases over consecutive seed ranges, first_seed on).  momentum > 0 (stacked
+    only) runs the trajectories with heavy-ball momentum."""
     net = Preconditioner(device=0, seed=1)
     if base:
         net.set_params(np.load(base, allow_pickle=False))
@@ -145,14 +146,16 @@ def train_study(out: str, phantoms: int = 48, train_steps: int = 4000, target: s
         if target != "best" or rounds:
             raise ValueError("stacked training takes target 'best' and no rounds")
         rep = L.train_stacked(net, 256, seeds, iters, stack=stack, frac=TTT_FRAC, train_steps=train_steps, batch=8,
-                              lr=1e-3, fp=TTT_FP, log=log)
+                              lr=1e-3, fp=TTT_FP, momentum=momentum, log=log)
     else:
+        if momentum:
+            raise ValueError("momentum trajectories take stacked training")
         rep = L.train(net, 256, seeds, iters, frac=TTT_FRAC, rounds=rounds, train_steps=train_steps, batch=8,
                       lr=1e-3, target=target, fp=TTT_FP, log=log)
     np.save(out, net.get_params(), allow_pickle=False)
     net.close()
     return {"study": "train", "out": out, "base": base, "first_seed": first_seed, "phantoms": rep.phantoms,
-            "iterations": iters, "stack": stack,
+            "iterations": iters, "stack": stack, "momentum": momentum,
             "samples": rep.samples,
             "loss_first": rep.losses[0], "loss_last50": float(np.mean(rep.losses[-50:])),
             "seconds": {k: round(v, 2) for k, v in rep.seconds.items()}, "total_seconds": round(time.perf_counter() - t, 2)}
