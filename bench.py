@@ -136,6 +136,31 @@ def cpu_baseline(c, ii, jj, fo, meas, threads, shots):
     return time.perf_counter() - t
 
 
+def _speedup_summary(sp, seeds):
+    """Speedups of one method to one target over the seeds (None: never
+    reached, counted as 0 in the median)."""
+    reached = [x for x in sp if x is not None]
+    return {"speedups": sp, "reached": f"{len(reached)} of {seeds}",
+            "at_least_3x": sum(1 for x in reached if x >= 3.0),
+            "median_speedup": float(np.median([x if x is not None else 0.0 for x in sp])) if sp else None}
+
+
+def ttt_summary(runs):
+    """The time-to-target study over its seeds: per target, the learned run's
+    speedups over plain SIRT and, when the runs carry momentum, those of
+    plain SIRT with the same momentum and no network.  Seeds whose plain run
+    never gets below 100 % are flagged and left out."""
+    out = {"study": "ttt", "runs": runs,
+           "flagged_never_below_100": [r["phantom_seed"] for r in runs if r["plain_never_below_100"]]}
+    counted = [r for r in runs if not r["plain_never_below_100"]]
+    for name in ("fixed50", "stop_rule", "fixed10"):
+        out[name] = _speedup_summary([r["targets"][name]["speedup"] for r in counted], len(counted))
+        if counted and counted[0]["momentum"]:
+            out[name]["momentum_plain"] = _speedup_summary(
+                [r["targets"][name]["momentum_plain_speedup"] for r in counted], len(counted))
+    return out
+
+
 def load_traffic(kernel, shape, tb):
     """HBM bytes per launch of `kernel` at this stencil shape and time
     blocking, from the committed counter summary (None when it has no row
@@ -346,24 +371,7 @@ def main():
                                       runs[-1]["targets"]["fixed50"]["momentum_plain_speedup"]}
                                      if runs[-1]["momentum"] else {})}),
                       file=sys.stderr, flush=True)
-            out = {"study": "ttt", "runs": runs, "flagged_never_below_100": [r["phantom_seed"] for r in runs
-                                                                             if r["plain_never_below_100"]]}
-            for name in ("fixed50", "stop_rule", "fixed10"):
-                counted = [r for r in runs if not r["plain_never_below_100"]]
-                sp = [r["targets"][name]["speedup"] for r in counted]
-                reached = [x for x in sp if x is not None]
-                # a seed whose learned run never reaches the target counts as a speedup of 0 in the median
-                out[name] = {"speedups": sp, "reached": f"{len(reached)} of {len(counted)}",
-                             "at_least_3x": sum(1 for x in reached if x >= 3.0),
-                             "median_speedup": float(np.median([x if x is not None else 0.0 for x in sp]))
-                             if sp else None}
-                if counted and counted[0]["momentum"]:
-                    # the same momentum without the network
-                    mp = [r["targets"][name]["momentum_plain_speedup"] for r in counted]
-                    out[name]["momentum_plain"] = {
-                        "speedups": mp, "reached": f"{sum(1 for x in mp if x is not None)} of {len(counted)}",
-                        "at_least_3x": sum(1 for x in mp if x is not None and x >= 3.0),
-                        "median_speedup": float(np.median([x if x is not None else 0.0 for x in mp]))}
+            out = ttt_summary(runs)
         print(json.dumps(out), flush=True)
         return
     if args.workload == "ref":

@@ -53,3 +53,31 @@ def test_roofline_figures_from_the_committed_counter_profile():
     assert abs(line["fractions"]["blocked_algorithmic"] - 12.0 * cells / 0.275e-3 / 8e12) < 1e-4
     assert abs(line["fractions"]["counter_measured"] - t_adj / 0.275e-3 / 8e12) < 1e-4
     assert line["bytes_per_cell_update"]["counter_measured"] == round(t_adj / cells, 3)
+
+
+def test_ttt_summary():
+    """The study's summary: a seed never reaching a target counts as 0 in the
+    median, a seed whose plain run never gets below 100 % is flagged and left
+    out, and the momentum-only speedups are summarised beside the learned
+    ones."""
+    bench = _bench_module()
+
+    def run_of(seed, never, f50, f10, m50=None, momentum=0.7):
+        t = {"fixed50": {"speedup": f50, "momentum_plain_speedup": m50},
+             "stop_rule": {"speedup": f50, "momentum_plain_speedup": m50},
+             "fixed10": {"speedup": f10, "momentum_plain_speedup": None}}
+        return {"phantom_seed": seed, "plain_never_below_100": never, "momentum": momentum, "targets": t}
+
+    runs = [run_of(1, False, 3.5, 2.0, 2.6), run_of(2, False, None, 1.5, 2.4), run_of(3, False, 1.2, 1.0, None),
+            run_of(4, True, 9.0, 9.0, 9.0)]
+    out = bench.ttt_summary(runs)
+    assert out["flagged_never_below_100"] == [4]
+    assert out["fixed50"]["speedups"] == [3.5, None, 1.2]
+    assert out["fixed50"]["reached"] == "2 of 3"
+    assert out["fixed50"]["at_least_3x"] == 1
+    assert out["fixed50"]["median_speedup"] == 1.2          # median of 3.5, 0, 1.2
+    assert out["fixed10"]["median_speedup"] == 1.5
+    mp = out["fixed50"]["momentum_plain"]
+    assert mp["speedups"] == [2.6, 2.4, None] and mp["reached"] == "2 of 3" and mp["median_speedup"] == 2.4
+    plain = bench.ttt_summary([run_of(1, False, 2.0, 2.0, momentum=0.0)])
+    assert "momentum_plain" not in plain["fixed50"]
