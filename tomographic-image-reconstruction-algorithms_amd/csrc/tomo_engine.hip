@@ -183,14 +183,6 @@ void destroy_spans(std::vector<TimedSpan> &v) {
   v.clear();
 }
 
-void graphs_clear(tomo_ctx *c) {
-  for (auto &g : c->graphs) {
-    if (g.exec) hipGraphExecDestroy(g.exec);
-    destroy_spans(g.spans);
-  }
-  c->graphs.clear();
-}
-
 // after a stream sync: fold recorded spans into the totals
 void spans_collect(tomo_ctx *c) {
   for (auto &s : c->spans) {
@@ -205,6 +197,27 @@ void spans_collect(tomo_ctx *c) {
     if (s.b) c->ev_free.push_back(s.b);
   }
   c->spans.clear();
+}
+
+// Spans of a replayed graph wait in c->spans until the call's closing sync
+// reads them; a call that failed part-way leaves them there.  They hold the
+// graph's own events, so they are read before any graph's events go.
+void collect_graph_spans(tomo_ctx *c) {
+  for (const auto &s : c->spans)
+    if (s.owned) {
+      (void)hipStreamSynchronize(c->stream);
+      spans_collect(c);
+      return;
+    }
+}
+
+void graphs_clear(tomo_ctx *c) {
+  collect_graph_spans(c);
+  for (auto &g : c->graphs) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    destroy_spans(g.spans);
+  }
+  c->graphs.clear();
 }
 
 }  // namespace
@@ -944,6 +957,7 @@ int gradient_pass_graph(tomo_ctx *c, int p0, int np, int P) {
   for (auto &g : c->graphs)
     if (g.p0 == p0 && g.np == np && g.P == P) ge = &g;
   if (ge && ge->sig != sig) {
+    collect_graph_spans(c);
     hipGraphExecDestroy(ge->exec);
     ge->exec = nullptr;
     destroy_spans(ge->spans);
@@ -1213,6 +1227,7 @@ void tomo_destroy(tomo_ctx *c) {
     if (s.a) hipEventDestroy(s.a);
     if (s.b) hipEventDestroy(s.b);
   }
+  c->spans.clear();
   for (auto e : c->ev_free) hipEventDestroy(e);
   graphs_clear(c);
   dfree(c->d_f); dfree(c->d_fo); dfree(c->d_pw); dfree(c->d_pw64);
