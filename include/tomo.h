@@ -207,7 +207,8 @@ int tomo_phantom_inclusions(float *fo, int nx, int ny, uint64_t seed, int count,
 /* Seeded smooth random velocity: white noise filtered by a Gaussian of
  * width sigma cells, scaled to a peak contrast of +-amp and tapered to zero
  * over sigma cells at the edge of [x0,x1) x [y0,y1); sets every cell of fo
- * to f = (1 + c)^2 - 1 (0 outside the region).  No reference counterpart. */
+ * to f = (1 + c)^2 - 1 (0 outside the region).  sigma must lie in
+ * (0, max(nx, ny)].  No reference counterpart. */
 int tomo_phantom_smooth(float *fo, int nx, int ny, uint64_t seed, double sigma, double amp,
                         int x0, int x1, int y0, int y1);
 

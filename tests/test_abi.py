@@ -86,6 +86,9 @@ def test_phantoms_are_seeded_and_bounded():
     c = np.sqrt(1 + s) - 1
     assert abs(np.abs(c).max() - 0.08) < 1e-6
     assert not s.reshape(n, n)[:32].any()
+    for bad_sigma in (0.0, -1.0, float(n) + 1.0, 1e12):  # no filter, or one wider than the grid
+        with pytest.raises(tomo.TomoError):
+            tomo.phantom_smooth(s, n, n, 7, bad_sigma, 0.08, region)
 
 
 def test_noise_rms_and_seed():

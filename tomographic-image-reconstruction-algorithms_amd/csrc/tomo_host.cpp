@@ -259,8 +259,10 @@ int tomo_phantom_inclusions(float *fo, int nx, int ny, uint64_t seed, int count,
 
 int tomo_phantom_smooth(float *fo, int nx, int ny, uint64_t seed, double sigma, double amp, int x0, int x1,
                         int y0, int y1) {
-  if (!fo || nx <= 0 || ny <= 0 || !(sigma > 0) || amp < 0 || x0 < 0 || y0 < 0 || x1 > nx || y1 > ny ||
-      x0 >= x1 || y0 >= y1)
+  // a filter wider than the grid (sigma above its larger side) is refused:
+  // its taps would outnumber the cells they act on
+  if (!fo || nx <= 0 || ny <= 0 || !(sigma > 0) || sigma > std::max(nx, ny) || amp < 0 || x0 < 0 || y0 < 0 ||
+      x1 > nx || y1 > ny || x0 >= x1 || y0 >= y1)
     return TOMO_ERR_ARG;
   Rng rng(seed);
   const size_t n2 = (size_t)nx * ny;
